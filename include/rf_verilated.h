@@ -1,0 +1,132 @@
+/* rf_verilated.h -- C ABI of the MI355X batch-stimulus simulation runtime
+ * (librf_verilated.so).
+ *
+ * Replaces the reference's generated model runtime, the ``RF::RTLflow`` class
+ * (src/V3EmitC.cpp:5246-5500) plus the ``Vtop`` model surface
+ * (src/V3EmitC.cpp:3510-3792), with a plain-C ABI that any host (C++,
+ * ctypes, cgo, JNI) can bind. A model owns all device memory of one design
+ * instantiated for ``n_lanes`` stimuli; one host thread per model.
+ *
+ * Errors: every entry point returns 0 on success or a negative RF_E* code
+ * and records a message readable with rf_last_error() (thread-local). The
+ * reference throws std::runtime_error on CUDA failures
+ * (src/V3EmitC.cpp:5327-5333) and aborts through VL_FATAL_MT on
+ * non-convergence (src/V3EmitC.cpp:5455-5462); neither crosses this ABI.
+ */
+#ifndef RF_VERILATED_H
+#define RF_VERILATED_H
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rf_design.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RF_OK = 0,
+    RF_E_ARG = -1,         /* bad argument (null, out of range, size mismatch) */
+    RF_E_HIP = -2,         /* HIP runtime error (allocation, launch, copy) */
+    RF_E_DESIGN = -3,      /* design table ABI mismatch */
+    RF_E_CONVERGE = -4,    /* settle loop hit the converge limit */
+    RF_E_STATE = -5        /* call not valid in the model's current state */
+};
+
+/* run_cycles modes */
+enum { RF_MODE_GRAPH = 0, RF_MODE_FUSED = 1 };
+
+typedef struct rf_config {
+    uint32_t settle_unroll;   /* graph form: settle iterations captured per eval (>= 1) */
+    uint32_t converge_limit;  /* settle iterations before RF_E_CONVERGE (reference: 100) */
+    uint32_t fused_chunk;     /* fused form: cycles per kernel launch */
+    uint32_t reserved;
+} rf_config;
+
+typedef struct rf_model rf_model;
+
+typedef struct rf_stats {
+    uint64_t fused_launches;  /* fused / persistent kernel launches */
+    double fused_ms;          /* their summed duration (HIP events on the model's stream) */
+    uint64_t graph_launches;  /* graph replays */
+    double graph_ms;          /* summed duration of graph replays (HIP events) */
+} rf_stats;
+
+/* Library / design ABI versions. */
+uint32_t rf_abi_version(void);
+
+/* Create a model: allocates the SoA state for n_lanes stimuli (zeroed: the
+ * --x-initial 0 reset), the per-lane change/done/hash/cycle/stimulus arrays,
+ * and builds the eval graph. Replaces RTLflow(size_t gpu_threads) +
+ * RTLflow::initialize(Vtop__Syms*) (src/V3EmitC.cpp:5294, :5351-5366,
+ * :5378-5497). cfg may be NULL for defaults. */
+int rf_create(const rf_design* design, uint32_t n_lanes, const rf_config* cfg, rf_model** out);
+
+/* Free everything (RTLflow::~RTLflow, src/V3EmitC.cpp:5367-5375). */
+int rf_destroy(rf_model* m);
+
+/* Port access for lanes [lane0, lane0+count). Host buffers hold, per lane,
+ * ``words`` elements of the port's storage class (u8/u16/u32/u64; u32 words
+ * for wide ports), lane-major. Replaces RTLflow::get(XDataLoc, idx) pointer
+ * writes/reads (src/V3EmitC.cpp:5339-5350). Any port may be read. */
+int rf_set_port(rf_model* m, uint32_t port, uint32_t lane0, uint32_t count, const void* src);
+int rf_get_port(rf_model* m, uint32_t port, uint32_t lane0, uint32_t count, void* dst);
+
+/* Memory (unpacked array) access for lanes [lane0, lane0+count), elements
+ * [elem0, elem0+nelem): host buffer [count][nelem][words] of the class type. */
+int rf_mem_write(rf_model* m, uint32_t mem, uint32_t lane0, uint32_t count, uint32_t elem0,
+                 uint32_t nelem, const void* src);
+int rf_mem_read(rf_model* m, uint32_t mem, uint32_t lane0, uint32_t count, uint32_t elem0,
+                uint32_t nelem, void* dst);
+
+/* Per-lane done mask (the reference's public ``done`` array, src/V3EmitC.cpp:5287-5292). */
+int rf_set_done(rf_model* m, uint32_t lane0, uint32_t count, const uint8_t* done);
+
+/* One eval() of every active lane with the settle loop (Vtop::eval ->
+ * RTLflow::run, src/V3EmitC.cpp:3715, :5376, :5426-5496). */
+int rf_eval(rf_model* m);
+
+/* Clock-cycle protocol with on-device stimulus and hashing:
+ * rf_begin sets each lane's hash to the FNV-1a-64 basis, its cycle counter to
+ * 0 and its stimulus id to stim0 + lane; rf_run_cycles runs n cycles
+ * (inputs for the lane's cycle with clk=0, eval, clk=1, eval, hash outputs).
+ * mode RF_MODE_GRAPH replays the captured per-mtask graph once per cycle,
+ * RF_MODE_FUSED runs the fused per-lane kernel. */
+int rf_begin(rf_model* m, uint64_t seed, uint32_t stim0);
+int rf_run_cycles(rf_model* m, uint32_t n, uint32_t mode);
+int rf_hashes(rf_model* m, uint32_t lane0, uint32_t count, uint64_t* out);
+
+/* Batched form: run nstim stimuli with the given cycle budgets through the
+ * model's lanes as a work queue in one persistent launch; each stimulus is
+ * reset and initialised when it enters a lane. order may be NULL (longest
+ * budget first). out_hashes[nstim] receives each stimulus's final hash. */
+int rf_submit_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
+                    const uint32_t* order, uint64_t* out_hashes);
+
+/* Per-stimulus memory images for the batched form: before a stimulus runs,
+ * elements [0, nelem) of memory ``mem`` in its lane are loaded from
+ * src[stimulus][nelem][words]. Pass src = NULL to clear. */
+int rf_set_mem_images(rf_model* m, uint32_t mem, uint32_t nstim, uint32_t nelem, const void* src);
+
+/* Raw state of one lane, every slot in class order C, S, I, Q then the
+ * lane-major memory region (for lane-k state diffs against a CPU model,
+ * the role of the reference's checkN asserter, src/V3EmitC.cpp:4380-4536).
+ * *bytes receives the size; dst may be NULL to query it. */
+int rf_lane_state(rf_model* m, uint32_t lane, void* dst, size_t* bytes);
+
+/* Wait for all queued work; device flags (RF_FLAG_*) of the last run. */
+int rf_sync(rf_model* m);
+int rf_flags(rf_model* m, uint32_t* out, uint32_t n);
+int rf_get_stats(rf_model* m, rf_stats* out);
+int rf_reset_stats(rf_model* m);
+
+/* Lanes and padded lane stride of the model. */
+uint32_t rf_lanes(const rf_model* m);
+uint64_t rf_lane_stride(const rf_model* m);
+
+const char* rf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

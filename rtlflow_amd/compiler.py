@@ -1,0 +1,935 @@
+"""Compile a design IR into a gfx950 HIP design library source.
+
+Pipeline (the reference's verilation stack, src/Verilator.cpp:116-537):
+ordering (order.py: V3Order/V3Clock/V3Changed) -> cost (instrcount.py or the
+learned model) -> mtask partition (partition.py: V3Partition) -> storage
+assignment (which variables live in SoA memory) -> layout (layout.py) -> kernels
+and descriptor (emit_hip.py + this module: V3EmitC).
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+from typing import Callable, Dict, List, Optional, Set
+
+from .emit_hip import CTY, Gen, cty, edge_cond, is_wide, lit
+from .instrcount import process_cost
+from .ir import CLASS_BYTES, Design, Ref, Var, words_of
+from .layout import Layout, class_of
+from .order import CompileError, Schedule, build_schedule
+from .partition import MTask, partition, partition_stats
+
+BLOCK = 256
+CLS_ID = {"C": 0, "S": 1, "I": 2, "Q": 3}
+
+
+class Options:
+    def __init__(self, threads: int = 8, block: int = BLOCK, lane_major_min_depth: int = 64,
+                 cost_fn: Optional[Callable] = None, settle_unroll: int = 2,
+                 converge_limit: int = 100, state_in_regs: bool = True, keep=None,
+                 mtask_stats: bool = False, inline_limit: int = 6000):
+        self.threads = threads            # --threads: partition granularity (cpLimit)
+        self.block = block
+        self.lane_major_min_depth = lane_major_min_depth
+        self.cost_fn = cost_fn            # vertex -> cost; default V3InstrCount restatement
+        self.settle_unroll = settle_unroll
+        self.converge_limit = converge_limit
+        self.state_in_regs = state_in_regs
+        self.keep = keep or set()
+        # designs whose summed logic cost exceeds this use the "calls" form
+        self.inline_limit = inline_limit
+
+    def key(self) -> str:
+        return f"t{self.threads}_b{self.block}_lm{self.lane_major_min_depth}"
+
+
+class Compiled:
+    host = False   # emit host functions (one-lane host build) instead of kernels
+
+    def __init__(self, design: Design, opts: Optional[Options] = None):
+        self.design = design
+        self.opts = opts or Options()
+        d = design
+        # synthetic clock-latch variables (__Vclklast__, src/V3Clock.cpp:64-92)
+        self.clklast: Dict[Var, Var] = {}
+        for clk in d.clocks:
+            v = Var(f"__Vclklast__{clk.name}", 1, False, "reg")
+            v.id = len(d.vars)
+            d.vars.append(v)
+            self.clklast[clk] = v
+        self.sch: Schedule = build_schedule(d, self.opts.keep)
+        sch = self.sch
+        # change-detection shadows (src/V3Changed.cpp:55-70)
+        self.chglast: Dict[Var, Var] = {}
+        for v in sch.change_vars:
+            s = Var(f"__Vchglast__{v.name}", v.width, False, "reg")
+            s.id = len(d.vars)
+            d.vars.append(s)
+            self.chglast[v] = s
+        # costs
+        self.vcost: List[int] = []
+        for vx in sch.vertices:
+            if self.opts.cost_fn is not None:
+                c = int(self.opts.cost_fn(self, vx))
+            else:
+                c = sum(process_cost(p) for p in vx.procs)
+            vx.cost = max(1, c)
+            self.vcost.append(vx.cost)
+        edges = [(vx.id, s) for vx in sch.vertices for s in vx.succs]
+        self.mtasks: List[MTask] = partition(self.vcost, edges, self.opts.threads, order=sch.topo)
+        self.vx2mt: Dict[int, int] = {}
+        for k, mt in enumerate(self.mtasks):
+            for vid in mt.vids:
+                self.vx2mt[vid] = k
+        self._storage()
+        self._layout()
+        # inline form (all logic in one function, state in registers) for
+        # small designs; shared non-inlined mtask bodies for large ones
+        self.calls = sum(self.vcost) > self.opts.inline_limit
+
+    # ---------------------------------------------------------- storage
+    def _storage(self):
+        d = self.design
+        sch = self.sch
+        live_vars: Set[Var] = set()
+        for vx in sch.vertices:
+            live_vars |= vx.reads | vx.writes | vx.need_old
+        P: Set[Var] = set()
+        for v in d.vars:
+            role = sch.roles.get(v)
+            if v.kind in ("input", "output"):
+                P.add(v)
+            elif v.is_mem and v in live_vars:
+                P.add(v)
+            elif role == "state" and v in live_vars:
+                P.add(v)
+        P |= set(self.clklast.values())
+        P |= set(sch.change_vars)
+        P |= set(self.chglast.values())
+        for vx in sch.vertices:
+            if vx.kind == "seq":
+                for v in vx.reads | vx.need_old:
+                    if sch.roles.get(v) == "comb":
+                        P.add(v)
+            else:
+                # latch-like comb vars (old value kept when not written)
+                for v in vx.need_old & vx.writes:
+                    P.add(v)
+        # initial processes may write vars; keep those that are live
+        for p in sch.initial:
+            for v in sch.access[p.id].writes:
+                if v in live_vars or v in P:
+                    P.add(v)
+        self.persistent = P
+        T: Set[Var] = set()
+        writer_mt: Dict[Var, Set[int]] = {}
+        for vx in sch.vertices:
+            for v in vx.writes:
+                writer_mt.setdefault(v, set()).add(self.vx2mt[vx.id])
+        for vx in sch.vertices:
+            mt = self.vx2mt[vx.id]
+            for v in vx.reads | vx.need_old:
+                ws = writer_mt.get(v, set())
+                if ws and (ws - {mt}):
+                    T.add(v)
+        for v, ws in writer_mt.items():
+            if len(ws) > 1:
+                T.add(v)
+        self.transfer = T - P
+        self.stored = P | T
+
+    def _layout(self):
+        d = self.design
+        lay = Layout()
+        order = []
+        order += [v for v in d.vars if v.kind == "input"]
+        order += [v for v in d.vars if v.kind == "output"]
+        order += [v for v in d.vars if v in self.stored and self.sch.roles.get(v) == "state"]
+        order += [v for v in d.vars if v in self.stored]
+        for v in order:
+            if v in self.stored or v.kind in ("input", "output"):
+                lay.add(v, lane_major=(v.is_mem and v.depth >= self.opts.lane_major_min_depth))
+        self.layout = lay
+
+    # ------------------------------------------------------- manifest
+    def manifest(self) -> dict:
+        d = self.design
+        lay = self.layout
+        seq_bytes = 0
+        for v in self.persistent:
+            if v.is_mem or v.kind == "input":
+                continue
+            if self.sch.roles.get(v) == "state" or v in self.clklast.values():
+                seq_bytes += v.slots * CLASS_BYTES[class_of(v)]
+        in_bytes = sum(v.slots * CLASS_BYTES[class_of(v)] for v in d.inputs if v not in d.clocks)
+        out_bytes = sum(v.slots * CLASS_BYTES[class_of(v)] for v in d.outputs)
+        st = partition_stats(self.mtasks)
+        return {
+            "name": d.name,
+            "slots": dict(lay.slots),
+            "lane_major_bytes": lay.lane_major_bytes,
+            "bytes_per_lane": lay.bytes_per_lane(),
+            "bytes_seq": seq_bytes,
+            "bytes_in": in_bytes,
+            "bytes_out": out_bytes,
+            "ops_per_cycle": 2 * sum(self.vcost),
+            "vertices": len(self.sch.vertices),
+            "mtasks": len(self.mtasks),
+            "partition": st,
+            "change_vars": [v.name for v in self.sch.change_vars],
+            "ports": [{k: x for k, x in p.items() if k != "var"} for p in self.port_table()],
+            "mems": self.mem_table(),
+            "vars": self.var_table(),
+            "threads": self.opts.threads,
+            "block": self.opts.block,
+            "form": "calls" if self.calls else "inline",
+        }
+
+    def mem_table(self) -> List[dict]:
+        out = []
+        for m in self.design.vars:
+            if not (m.is_mem and self.layout.has(m)):
+                continue
+            lm = m in self.layout.lane_major
+            out.append({"name": m.name, "width": m.width, "cls": CLS_ID[class_of(m)],
+                        "words": m.words, "depth": m.depth, "lane_major": lm,
+                        "loc": self.layout.lane_major[m] if lm else self.layout.loc[m]})
+        return out
+
+    def var_table(self) -> List[dict]:
+        """Stored scalar/wide variables and their slots (for lane state diffs)."""
+        out = []
+        for v in self.design.vars:
+            if v.is_mem or v not in self.layout.loc:
+                continue
+            out.append({"name": v.name, "width": v.width, "cls": CLS_ID[class_of(v)],
+                        "words": v.words, "loc": self.layout.loc[v],
+                        "role": self.sch.roles.get(v, "internal")})
+        return out
+
+    def port_table(self) -> List[dict]:
+        d = self.design
+        out = []
+        stim_idx = 0
+        for v in d.vars:
+            if v.kind not in ("input", "output"):
+                continue
+            if v.kind == "input":
+                if v in d.clocks:
+                    dirn = 2
+                    si = 0xffffffff
+                else:
+                    dirn = 0
+                    si = stim_idx
+                    stim_idx += 1
+            else:
+                dirn = 1
+                si = 0xffffffff
+            out.append({"name": v.name, "dir": dirn, "width": v.width, "cls": CLS_ID[class_of(v)],
+                        "words": v.slots, "loc": self.layout.loc[v], "stim_index": si,
+                        "hashed": 1 if v.kind == "output" else 0, "var": v})
+        return out
+
+
+# =====================================================================
+# Kernel emission
+# =====================================================================
+
+def kernel_header(comp: Compiled, name: str) -> str:
+    """Device kernel, or for the host build a per-lane C function with the
+    same body."""
+    if comp.host:
+        return f'extern "C" void {name}(rf_args a, uint32_t lane_in) {{'
+    return f'extern "C" __global__ void __launch_bounds__({comp.opts.block}) {name}(rf_args a) {{'
+
+
+def lane_expr(comp: Compiled) -> str:
+    return "lane_in" if comp.host else f"blockIdx.x * {comp.opts.block}u + threadIdx.x"
+
+
+def atomic(comp: Compiled, op: str, ptr: str, val: str) -> str:
+    if comp.host:
+        return f"__atomic_fetch_{op}({ptr}, {val}, __ATOMIC_RELAXED)"
+    return f"atomic{op.capitalize()}({ptr}, {val})"
+
+
+def wave_or_flag(comp: Compiled, cond: str, ptr: str) -> str:
+    """OR a per-lane condition into one device word: one atomic per wavefront."""
+    if comp.host:
+        return f"if ({cond}) __atomic_fetch_or({ptr}, 1u, __ATOMIC_RELAXED);"
+    return f"if (__any({cond}) && (threadIdx.x & 63u) == 0u) atomicOr({ptr}, 1u);"
+
+
+def _prologue(comp: Compiled, gen: Gen, lines: List[str], check_done=True, check_change=False):
+    B = comp.opts.block
+    lines.append(f"    const uint32_t lane = {lane_expr(comp)};")
+    lines.append("    if (lane >= a.nlanes) return;")
+    if check_done:
+        lines.append("    if (a.done[lane]) return;")
+    if check_change:
+        lines.append("    if (!a.change[lane]) return;")
+    lines.append("    const size_t NP = a.npad;")
+    lines.append("    uint8_t* __restrict__ sC = a.c + lane;")
+    lines.append("    uint16_t* __restrict__ sS = a.s + lane;")
+    lines.append("    uint32_t* __restrict__ sI = a.i + lane;")
+    lines.append("    uint64_t* __restrict__ sQ = a.q + lane;")
+    lay = comp.layout
+    for m, off in lay.lane_major.items():
+        c = class_of(m)
+        row = m.depth * (CLASS_BYTES[m.cls] if m.cls != "W" else 4 * m.words)
+        lines.append(f"    {CTY[c]}* __restrict__ mb{m.id} = ({CTY[c]}*)(a.mem + (size_t){off} * NP"
+                     f" + (size_t)lane * {row}u);")
+    lines.append("    (void)sC; (void)sS; (void)sI; (void)sQ;")
+
+
+def _vertex_vars(comp: Compiled, vids: List[int]) -> Set[Var]:
+    out: Set[Var] = set()
+    for vid in vids:
+        vx = comp.sch.vertices[vid]
+        for p in vx.procs:
+            acc = comp.sch.access[p.id]
+            out |= acc.dep_reads | acc.need_old | acc.writes
+    return {v for v in out if not v.is_mem}
+
+
+def _emit_vertices(comp: Compiled, gen: Gen, vids: List[int], store_in_guard: bool):
+    sch = comp.sch
+    for vid in vids:
+        vx = sch.vertices[vid]
+        if vx.kind == "seq":
+            edges = []
+            for p in vx.procs:
+                for e in p.edges:
+                    if e not in edges:
+                        edges.append(e)
+            cond = edge_cond(gen, edges)
+            gen.emit(f"if ({cond}) {{")
+            gen.ind += 1
+            saved = set(gen.avail)
+            saved_dirty = set(gen.dirty)
+            for p in vx.procs:
+                gen.process(p, sch.access[p.id], f"{p.id}")
+            if store_in_guard:
+                written = set()
+                for p in vx.procs:
+                    written |= {v for v in sch.access[p.id].writes if not v.is_mem}
+                for v in sorted(written, key=lambda x: x.id):
+                    if v not in saved and v in comp.stored:
+                        gen.store(v)
+                        gen.dirty.discard(v)
+            gen.ind -= 1
+            gen.emit("}")
+            gen.avail = saved
+            if store_in_guard:
+                gen.dirty = {v for v in gen.dirty if v in saved or v in saved_dirty}
+        else:
+            for p in vx.procs:
+                gen.process(p, sch.access[p.id], f"{p.id}")
+
+
+def _mtask_gen(comp: Compiled, k: int) -> Gen:
+    mt = comp.mtasks[k]
+    gen = Gen(comp)
+    gen.top_lines = []
+    gen.local = _vertex_vars(comp, mt.vids)
+    # clocks and clock latches used by edge guards
+    for vid in mt.vids:
+        vx = comp.sch.vertices[vid]
+        for p in vx.procs:
+            for _, clk in p.edges:
+                gen.local.add(clk)
+                gen.local.add(comp.clklast[clk])
+    _emit_vertices(comp, gen, mt.vids, store_in_guard=True)
+    for v in sorted(gen.dirty, key=lambda x: x.id):
+        if v in comp.stored:
+            gen.store(v)
+    return gen
+
+
+def emit_mtask_kernel(comp: Compiled, k: int) -> str:
+    if comp.calls:
+        # thin wrapper over the shared mtask body
+        lines = [kernel_header(comp, f"rf_mtask_{k}")]
+        lines.append(f"    const uint32_t lane = {lane_expr(comp)};")
+        lines.append("    if (lane >= a.nlanes) return;")
+        lines.append("    if (a.done[lane]) return;")
+        if comp.sch.change_vars:
+            lines.append("    if (!a.change[lane]) return;")
+        lines.append(f"    rf_body_{k}(a, lane);")
+        lines.append("}")
+        return "\n".join(lines)
+    gen = _mtask_gen(comp, k)
+    lines = [kernel_header(comp, f"rf_mtask_{k}")]
+    _prologue(comp, gen, lines, check_done=True, check_change=bool(comp.sch.change_vars))
+    lines += gen.top_lines
+    lines += gen.lines
+    lines.append("}")
+    return "\n".join(lines)
+
+
+# ---------------------------------------------------------------------------
+# "calls" form for large designs: each mtask body is one non-inlined device
+# function shared by its graph-node kernel and by the init / fused /
+# persistent kernels, which call the bodies in topological order; values cross
+# bodies through the SoA state (which the caches keep close).
+# ---------------------------------------------------------------------------
+
+def body_header(comp: Compiled, name: str, ret: str = "void", extra: str = "") -> str:
+    q = "static" if comp.host else "__device__ __attribute__((noinline))"
+    return f"{q} {ret} {name}(const rf_args& a, uint32_t lane{extra}) {{"
+
+
+def _body_prologue(comp: Compiled, lines: List[str]):
+    lines.append("    const size_t NP = a.npad;")
+    lines.append("    uint8_t* __restrict__ sC = a.c + lane;")
+    lines.append("    uint16_t* __restrict__ sS = a.s + lane;")
+    lines.append("    uint32_t* __restrict__ sI = a.i + lane;")
+    lines.append("    uint64_t* __restrict__ sQ = a.q + lane;")
+    for m, off in comp.layout.lane_major.items():
+        c = class_of(m)
+        row = m.depth * (CLASS_BYTES[m.cls] if m.cls != "W" else 4 * m.words)
+        lines.append(f"    {CTY[c]}* __restrict__ mb{m.id} = ({CTY[c]}*)(a.mem + (size_t){off} * NP"
+                     f" + (size_t)lane * {row}u);")
+    lines.append("    (void)sC; (void)sS; (void)sI; (void)sQ; (void)NP;")
+
+
+def _wrap_body(comp: Compiled, gen: Gen, name: str, ret: str = "void", extra: str = "",
+               tail: Optional[List[str]] = None) -> str:
+    lines = [body_header(comp, name, ret, extra)]
+    _body_prologue(comp, lines)
+    lines += gen.top_lines + gen.lines + (tail or []) + ["}"]
+    return "\n".join(lines)
+
+
+def emit_bodies(comp: Compiled) -> List[str]:
+    d = comp.design
+    out = []
+    for k in range(len(comp.mtasks)):
+        out.append(_wrap_body(comp, _mtask_gen(comp, k), f"rf_body_{k}"))
+    # latch + change request -> returns the lane's change flag
+    gen = Gen(comp)
+    gen.top_lines = []
+    gen.local = set(d.clocks) | set(comp.clklast.values()) | set(comp.sch.change_vars) | \
+        set(comp.chglast.values())
+    for clk, last in comp.clklast.items():
+        gen.ensure(clk)
+        gen.declare(last)
+        gen.emit(f"{gen.name(last)} = {gen.name(clk)};")
+        gen.avail.add(last)
+        gen.dirty.add(last)
+    if comp.sch.change_vars:
+        _change_code(comp, gen, "ch")
+    else:
+        gen.emit("const uint32_t ch = 0u;")
+    for v in sorted(gen.dirty, key=lambda x: x.id):
+        gen.store(v)
+    out.append(_wrap_body(comp, gen, "rf_body_latch", "uint32_t", tail=["    return ch;"]))
+    # stimulus for (sid, cyc) and clk = 0
+    gen = Gen(comp)
+    gen.top_lines = []
+    gen.local = set(d.inputs)
+    _stim_code(comp, gen, "sid", "cyc")
+    for v in sorted(gen.dirty, key=lambda x: x.id):
+        gen.store(v)
+    out.append(_wrap_body(comp, gen, "rf_body_stim", extra=", uint64_t sid, uint64_t cyc"))
+    # clock value
+    gen = Gen(comp)
+    gen.top_lines = []
+    gen.local = set(d.clocks)
+    for clk in d.clocks:
+        gen.declare(clk)
+        gen.emit(f"{gen.name(clk)} = v;")
+        gen.store(clk)
+    out.append(_wrap_body(comp, gen, "rf_body_setclk", extra=", uint32_t v"))
+    # output hash
+    gen = Gen(comp)
+    gen.top_lines = []
+    gen.local = set(d.outputs)
+    _hash_code(comp, gen, "h")
+    out.append(_wrap_body(comp, gen, "rf_body_hash", "uint64_t", extra=", uint64_t h",
+                          tail=["    return h;"]))
+    # initial blocks + clock latch
+    gen = Gen(comp)
+    gen.top_lines = []
+    sch = comp.sch
+    loc = set(d.clocks) | set(comp.clklast.values())
+    for p in sch.initial:
+        acc = sch.access[p.id]
+        loc |= {v for v in acc.dep_reads | acc.need_old | acc.writes if not v.is_mem}
+    gen.local = loc
+    for p in sch.initial:
+        gen.process(p, sch.access[p.id], f"i{p.id}")
+    for clk, last in comp.clklast.items():
+        gen.ensure(clk)
+        gen.declare(last)
+        gen.emit(f"{gen.name(last)} = {gen.name(clk)};")
+        gen.avail.add(last)
+        gen.dirty.add(last)
+    for v in sorted(gen.dirty, key=lambda x: x.id):
+        if v in comp.stored:
+            gen.store(v)
+    out.append(_wrap_body(comp, gen, "rf_body_initial"))
+    return out
+
+
+def _calls_eval(comp: Compiled, ind: str) -> List[str]:
+    """One eval through the bodies, with the settle loop."""
+    calls = [f"{ind}    rf_body_{k}(a, lane);" for k in range(len(comp.mtasks))]
+    lim = atomic(comp, "or", "&a.flags[RF_FLAG_UNCONVERGED]", "1u")
+    if not comp.sch.change_vars:
+        return calls + [f"{ind}    (void)rf_body_latch(a, lane);"]
+    out = [f"{ind}for (uint32_t it = 0u;; ++it) {{"]
+    out += [("    " + c) for c in calls]
+    out.append(f"{ind}    if (!rf_body_latch(a, lane)) break;")
+    out.append(f"{ind}    if (it + 1u >= a.converge_limit) {{ {lim}; break; }}")
+    out.append(f"{ind}}}")
+    return out
+
+
+def emit_calls_kernels(comp: Compiled) -> List[str]:
+    lay = comp.layout
+    out = []
+    B = comp.opts.block
+    # init: initial blocks, latch, settle
+    lines = [kernel_header(comp, "rf_k_init"),
+             f"    const uint32_t lane = {lane_expr(comp)};",
+             "    if (lane >= a.nlanes) return;",
+             "    rf_body_initial(a, lane);"]
+    lines += _calls_eval(comp, "    ")
+    lines.append("}")
+    out.append("\n".join(lines))
+    # fused protocol kernel
+    lines = [kernel_header(comp, "rf_k_fused"),
+             f"    const uint32_t lane = {lane_expr(comp)};",
+             "    if (lane >= a.nlanes) return;",
+             "    if (a.done[lane]) return;",
+             "    uint64_t h = a.hash[lane];",
+             "    uint32_t cyc = a.cyc[lane];",
+             "    const uint64_t sid = (uint64_t)a.stim[lane];",
+             "    for (uint32_t n = 0u; n < a.ncycles; ++n, ++cyc) {",
+             "        rf_body_stim(a, lane, sid, (uint64_t)cyc);",
+             "        for (uint32_t ph = 0u; ph < 2u; ++ph) {",
+             "            rf_body_setclk(a, lane, ph);"]
+    lines += _calls_eval(comp, "            ")
+    lines += ["        }",
+              "        h = rf_body_hash(a, lane, h);",
+              "    }",
+              "    a.hash[lane] = h;",
+              "    a.cyc[lane] = cyc;",
+              "}"]
+    out.append("\n".join(lines))
+    # persistent batched kernel
+    lines = [kernel_header(comp, "rf_k_persistent"),
+             f"    const uint32_t lane = {lane_expr(comp)};",
+             "    if (lane >= a.nlanes) return;",
+             "    const size_t NP = a.npad;",
+             "    uint32_t q = lane;",
+             "    while (q < a.nstim) {",
+             "        const uint32_t sid = a.order ? a.order[q] : q;",
+             "        const uint32_t budget = a.budget[sid];"]
+    for c, arr in (("C", "a.c"), ("S", "a.s"), ("I", "a.i"), ("Q", "a.q")):
+        if lay.slots[c]:
+            lines.append(f"        for (size_t k = 0; k < {lay.slots[c]}u; ++k) {arr}[k * NP + lane] = 0;")
+    for m, off in lay.lane_major.items():
+        row = m.depth * (CLASS_BYTES[m.cls] if m.cls != "W" else 4 * m.words)
+        lines.append(f"        {{ uint64_t* p = (uint64_t*)(a.mem + (size_t){off} * NP + (size_t)lane * {row}u);")
+        lines.append(f"          for (uint32_t k = 0; k < {row // 8}u; ++k) p[k] = 0ull; }}")
+    lines.append("        rf_body_initial(a, lane);")
+    lines += _calls_eval(comp, "        ")
+    lines += ["        uint64_t h = RF_FNV_BASIS;",
+              "        for (uint32_t cyc = 0u; cyc < budget; ++cyc) {",
+              "            rf_body_stim(a, lane, (uint64_t)sid, (uint64_t)cyc);",
+              "            for (uint32_t ph = 0u; ph < 2u; ++ph) {",
+              "                rf_body_setclk(a, lane, ph);"]
+    lines += _calls_eval(comp, "                ")
+    lines += ["            }",
+              "            h = rf_body_hash(a, lane, h);",
+              "        }",
+              "        a.out_hash[sid] = h;",
+              f"        q = {atomic(comp, 'add', '&a.flags[RF_FLAG_QUEUE]', '1u')};",
+              "    }",
+              "}"]
+    out.append("\n".join(lines))
+    return out
+
+
+def _change_code(comp: Compiled, gen: Gen, result: str):
+    """change = OR (x != x_old); x_old = x (src/V3Changed.cpp:55-70, 313-316)."""
+    gen.emit(f"uint32_t {result} = 0u;")
+    for v in comp.sch.change_vars:
+        old = comp.chglast[v]
+        gen.ensure(v)
+        gen.ensure(old)
+        if is_wide(v.width):
+            gen.emit(f"{result} |= rfw_eq({gen.name(v)}, {gen.name(old)}, {v.width}) ^ 1u;")
+            gen.emit(f"rfw_copy({gen.name(old)}, {gen.name(v)}, {v.width});")
+        else:
+            gen.emit(f"{result} |= (uint32_t)({gen.name(v)} != {gen.name(old)});")
+            gen.emit(f"{gen.name(old)} = {gen.name(v)};")
+        gen.dirty.add(old)
+
+
+def _hash_code(comp: Compiled, gen: Gen, h: str):
+    for port in comp.port_table():
+        if not port["hashed"]:
+            continue
+        v = port["var"]
+        gen.ensure(v)
+        c = class_of(v)
+        if is_wide(v.width):
+            for w in range(v.words):
+                gen.emit(f"{h} = rf_fnv_bytes({h}, (uint64_t){gen.name(v)}[{w}], 4);")
+        else:
+            gen.emit(f"{h} = rf_fnv_bytes({h}, (uint64_t){gen.name(v)}, {CLASS_BYTES[c]});")
+
+
+def _stim_code(comp: Compiled, gen: Gen, sid: str, cyc: str):
+    for port in comp.port_table():
+        if port["dir"] != 0:
+            continue
+        v = port["var"]
+        gen.declare(v)
+        p = port["stim_index"]
+        n = v.name
+        if is_wide(v.width):
+            for j in range((v.width + 63) // 64):
+                gen.emit(f"{{ const uint64_t r = rf_stim(a.seed, {sid}, {cyc}, {(p << 3) | j}u);")
+                gen.emit(f"  {gen.name(v)}[{2 * j}] = (uint32_t)r;")
+                if 2 * j + 1 < v.words:
+                    gen.emit(f"  {gen.name(v)}[{2 * j + 1}] = (uint32_t)(r >> 32); }}")
+                else:
+                    gen.emit("}")
+            gen.emit(f"rfw_clean({gen.name(v)}, {v.width});")
+        else:
+            gen.emit(f"{gen.name(v)} = ({cty(v.width)})(rf_stim(a.seed, {sid}, {cyc}, {p << 3}u) & "
+                     f"{lit((1 << v.width) - 1, 64)});")
+        gen.avail.add(v)
+        gen.dirty.add(v)
+
+
+def emit_eval_end(comp: Compiled) -> str:
+    d = comp.design
+    gen = Gen(comp)
+    gen.top_lines = []
+    gen.local = set(d.clocks) | set(comp.clklast.values()) | set(comp.sch.change_vars) | \
+        set(comp.chglast.values()) | set(d.outputs)
+    body = gen
+    # _last_assign
+    for clk, last in comp.clklast.items():
+        gen.ensure(clk)
+        gen.declare(last)
+        gen.emit(f"{gen.name(last)} = {gen.name(clk)};")
+        gen.avail.add(last)
+        gen.dirty.add(last)
+    if comp.sch.change_vars:
+        _change_code(comp, gen, "ch")
+        gen.emit("a.change[lane] = ch;")
+        gen.emit(wave_or_flag(comp, "ch != 0u", "&a.flags[RF_FLAG_CHANGE]"))
+        gen.emit(f"if (ch && a.phase != RF_PHASE_EVAL) {atomic(comp, 'or', '&a.flags[RF_FLAG_UNCONVERGED]', '1u')};")
+        gen.emit("if (a.phase != RF_PHASE_EVAL) a.change[lane] = 1u;")
+    gen.emit("if (a.phase == RF_PHASE_RISE) {")
+    gen.ind += 1
+    for clk in d.clocks:
+        gen.declare(clk)
+        gen.emit(f"{gen.name(clk)} = 1u;")
+        gen.dirty.add(clk)
+    gen.ind -= 1
+    gen.emit("} else if (a.phase == RF_PHASE_HASH) {")
+    gen.ind += 1
+    gen.emit("uint64_t h = a.hash[lane];")
+    _hash_code(comp, gen, "h")
+    gen.emit("a.hash[lane] = h;")
+    gen.emit("a.cyc[lane] += 1u;")
+    gen.ind -= 1
+    gen.emit("}")
+    for v in sorted(gen.dirty, key=lambda x: x.id):
+        gen.store(v)
+    lines = [kernel_header(comp, f"rf_k_eval_end")]
+    _prologue(comp, gen, lines, check_done=True)
+    lines += gen.top_lines + gen.lines + ["}"]
+    return "\n".join(lines)
+
+
+def emit_stim(comp: Compiled) -> str:
+    d = comp.design
+    gen = Gen(comp)
+    gen.top_lines = []
+    gen.local = set(d.inputs)
+    _stim_code(comp, gen, "(uint64_t)a.stim[lane]", "(uint64_t)a.cyc[lane]")
+    for clk in d.clocks:
+        gen.declare(clk)
+        gen.emit(f"{gen.name(clk)} = 0u;")
+        gen.dirty.add(clk)
+    if comp.sch.change_vars:
+        gen.emit("a.change[lane] = 1u;")
+    for v in sorted(gen.dirty, key=lambda x: x.id):
+        gen.store(v)
+    lines = [kernel_header(comp, f"rf_k_stim")]
+    _prologue(comp, gen, lines, check_done=True)
+    lines += gen.top_lines + gen.lines + ["}"]
+    return "\n".join(lines)
+
+
+def _all_comb_vids(comp: Compiled) -> List[int]:
+    return [vid for vid in comp.sch.topo if comp.sch.vertices[vid].kind == "comb"]
+
+
+def _initial_body(comp: Compiled, gen: Gen):
+    """Initial blocks, clock latch, then the settle loop (src/V3Clock.cpp:228-270)."""
+    sch = comp.sch
+    for p in sch.initial:
+        gen.process(p, sch.access[p.id], f"i{p.id}")
+    for clk, last in comp.clklast.items():
+        gen.ensure(clk)
+        gen.declare(last)
+        gen.emit(f"{gen.name(last)} = {gen.name(clk)};")
+        gen.avail.add(last)
+        gen.dirty.add(last)
+    comb = _all_comb_vids(comp)
+    if comp.sch.change_vars:
+        gen.emit("uint32_t it = 0u;")
+        # preload every variable the loop body touches so the loop sees a
+        # consistent register state on its back edge
+        for v in sorted(_vertex_vars(comp, comb) | set(comp.chglast.values()) |
+                        set(comp.sch.change_vars), key=lambda x: x.id):
+            if v in comp.stored:
+                gen.ensure(v)
+            else:
+                gen.declare(v)
+                gen.avail.add(v)
+        gen.emit("for (;;) {")
+        gen.ind += 1
+        _emit_vertices(comp, gen, comb, store_in_guard=False)
+        _change_code(comp, gen, "ch")
+        gen.emit(f"if (!ch) break;")
+        gen.emit(f"if (++it >= a.converge_limit) {{ {atomic(comp, 'or', '&a.flags[RF_FLAG_UNCONVERGED]', '1u')}; break; }}")
+        gen.ind -= 1
+        gen.emit("}")
+    else:
+        _emit_vertices(comp, gen, comb, store_in_guard=False)
+
+
+def emit_init(comp: Compiled) -> str:
+    gen = Gen(comp)
+    gen.top_lines = []
+    sch = comp.sch
+    loc = _vertex_vars(comp, list(range(len(sch.vertices))))
+    for p in sch.initial:
+        acc = sch.access[p.id]
+        loc |= {v for v in acc.dep_reads | acc.need_old | acc.writes if not v.is_mem}
+    loc |= set(comp.design.clocks) | set(comp.clklast.values()) | set(comp.chglast.values()) | \
+        set(sch.change_vars)
+    gen.local = loc
+    _initial_body(comp, gen)
+    for v in sorted(gen.dirty, key=lambda x: x.id):
+        if v in comp.stored:
+            gen.store(v)
+    lines = [kernel_header(comp, f"rf_k_init")]
+    _prologue(comp, gen, lines, check_done=False)
+    lines += gen.top_lines + gen.lines + ["}"]
+    return "\n".join(lines)
+
+
+def _eval_loop_body(comp: Compiled, gen: Gen):
+    """One eval() with the settle loop on device (src/V3EmitC.cpp:5426-5496)."""
+    all_vids = list(comp.sch.topo)
+    if comp.sch.change_vars:
+        gen.emit("for (uint32_t it = 0u;; ++it) {")
+        gen.ind += 1
+    _emit_vertices(comp, gen, all_vids, store_in_guard=False)
+    for clk, last in comp.clklast.items():
+        gen.emit(f"{gen.name(last)} = {gen.name(clk)};")
+        gen.dirty.add(last)
+    if comp.sch.change_vars:
+        _change_code(comp, gen, "ch")
+        gen.emit("if (!ch) break;")
+        gen.emit(f"if (it + 1u >= a.converge_limit) {{ {atomic(comp, 'or', '&a.flags[RF_FLAG_UNCONVERGED]', '1u')}; break; }}")
+        gen.ind -= 1
+        gen.emit("}")
+
+
+def _fused_locals(comp: Compiled) -> Set[Var]:
+    sch = comp.sch
+    loc = _vertex_vars(comp, list(range(len(sch.vertices))))
+    loc |= set(comp.design.clocks) | set(comp.clklast.values()) | set(comp.chglast.values()) | \
+        set(sch.change_vars) | set(comp.design.inputs) | set(comp.design.outputs)
+    return loc
+
+
+def emit_fused(comp: Compiled) -> str:
+    """Protocol kernel: ``a.ncycles`` cycles per lane with all logic inlined."""
+    d = comp.design
+    gen = Gen(comp)
+    gen.top_lines = []
+    gen.local = _fused_locals(comp)
+    # registers that carry values across evals are loaded once
+    for v in sorted(gen.local, key=lambda x: x.id):
+        if v in comp.persistent:
+            gen.ensure(v)
+        else:
+            gen.declare(v)
+            gen.avail.add(v)
+    gen.emit("uint64_t h = a.hash[lane];")
+    gen.emit("uint32_t cyc = a.cyc[lane];")
+    gen.emit("const uint64_t sid = (uint64_t)a.stim[lane];")
+    gen.emit("for (uint32_t n = 0u; n < a.ncycles; ++n, ++cyc) {")
+    gen.ind += 1
+    _stim_code(comp, gen, "sid", "(uint64_t)cyc")
+    gen.emit("#pragma unroll 1")
+    gen.emit("for (uint32_t ph = 0u; ph < 2u; ++ph) {")
+    gen.ind += 1
+    for clk in d.clocks:
+        gen.emit(f"{gen.name(clk)} = ph;")
+        gen.dirty.add(clk)
+    _eval_loop_body(comp, gen)
+    gen.ind -= 1
+    gen.emit("}")
+    _hash_code(comp, gen, "h")
+    gen.ind -= 1
+    gen.emit("}")
+    gen.emit("a.hash[lane] = h;")
+    gen.emit("a.cyc[lane] = cyc;")
+    for v in sorted(gen.dirty | (gen.local & comp.stored), key=lambda x: x.id):
+        if v in comp.stored:
+            gen.store(v)
+    lines = [kernel_header(comp, f"rf_k_fused")]
+    _prologue(comp, gen, lines, check_done=True)
+    lines += gen.top_lines + gen.lines + ["}"]
+    return "\n".join(lines)
+
+
+def emit_reset(comp: Compiled) -> str:
+    """Zero every slot of one lane (variable reset with --x-initial 0, the
+    zeroing path of the reference's _ctor_var_reset, src/V3EmitC.cpp:2042-2047)."""
+    lay = comp.layout
+    lines = [kernel_header(comp, f"rf_k_reset")]
+    lines.append(f"    const uint32_t lane = {lane_expr(comp)};")
+    lines.append("    if (lane >= a.nlanes) return;")
+    lines.append("    const size_t NP = a.npad;")
+    for c, arr in (("C", "a.c"), ("S", "a.s"), ("I", "a.i"), ("Q", "a.q")):
+        n = lay.slots[c]
+        if n:
+            lines.append(f"    for (size_t k = 0; k < {n}u; ++k) {arr}[k * NP + lane] = 0;")
+    if lay.lane_major_bytes:
+        lines.append(f"    uint64_t* row = (uint64_t*)(a.mem + (size_t)lane * 0);")
+        for m, off in lay.lane_major.items():
+            row = m.depth * (CLASS_BYTES[m.cls] if m.cls != "W" else 4 * m.words)
+            lines.append(f"    {{ uint8_t* p = a.mem + (size_t){off} * NP + (size_t)lane * {row}u;")
+            lines.append(f"      for (uint32_t k = 0; k < {row}u; ++k) p[k] = 0; }}")
+    lines.append("}")
+    return "\n".join(lines)
+
+
+def emit_source(comp: Compiled, host: bool = False) -> str:
+    """The design library source: gfx950 kernels, or (host=True) the same
+    bodies as per-lane host functions for the one-lane host build."""
+    comp.host = host
+    try:
+        return _emit_source(comp)
+    finally:
+        comp.host = False
+
+
+def _emit_source(comp: Compiled) -> str:
+    d = comp.design
+    parts = [
+        "// Generated by rtlflow_amd " + ("(host build)" if comp.host else "for gfx950") +
+        ". Do not edit.",
+        f"// design: {d.name}",
+        "" if comp.host else '#include <hip/hip_runtime.h>',
+        '#include "rf_design.h"',
+        '#include "rf_ops.h"',
+        '#include "rf_stim.h"',
+        "",
+    ]
+    if comp.calls:
+        for b in emit_bodies(comp):
+            parts.append(b)
+            parts.append("")
+    for k in range(len(comp.mtasks)):
+        parts.append(emit_mtask_kernel(comp, k))
+        parts.append("")
+    parts.append(emit_eval_end(comp))
+    parts.append("")
+    parts.append(emit_stim(comp))
+    parts.append("")
+    parts.append(emit_reset(comp))
+    parts.append("")
+    if comp.calls:
+        for kx in emit_calls_kernels(comp):
+            parts.append(kx)
+            parts.append("")
+    else:
+        parts.append(emit_init(comp))
+        parts.append("")
+        parts.append(emit_fused(comp))
+        parts.append("")
+        from .persistent import emit_persistent
+        parts.append(emit_persistent(comp))
+        parts.append("")
+    body = "\n".join(parts)
+    h = int(hashlib.sha1(body.encode()).hexdigest()[:16], 16)
+    parts = [body, emit_descriptor(comp, h)]
+    return "\n".join(parts)
+
+
+def emit_descriptor(comp: Compiled, h: int) -> str:
+    d = comp.design
+    man = comp.manifest()
+    out = []
+    ports = comp.port_table()
+    out.append("static const rf_port rf_ports[] = {")
+    for p in ports:
+        out.append(f'    {{"{p["name"]}", {p["dir"]}u, {p["width"]}u, {p["cls"]}u, {p["words"]}u, '
+                   f'{p["loc"]}ull, {p["stim_index"]}u, {p["hashed"]}u}},')
+    if not ports:
+        out.append("    {0}")
+    out.append("};")
+    mems = [v for v in d.vars if v.is_mem and comp.layout.has(v)]
+    out.append("static const rf_mem rf_mems[] = {")
+    for m in mems:
+        lm = m in comp.layout.lane_major
+        loc = comp.layout.lane_major[m] if lm else comp.layout.loc[m]
+        row = m.depth * (CLASS_BYTES[m.cls] if m.cls != "W" else 4 * m.words)
+        out.append(f'    {{"{m.name}", {m.width}u, {CLS_ID[class_of(m)]}u, {m.words}u, {m.depth}u, '
+                   f'{1 if lm else 0}u, {loc}ull, {row}ull}},')
+    if not mems:
+        out.append("    {0}")
+    out.append("};")
+    for k, mt in enumerate(comp.mtasks):
+        preds = sorted(comp.mtasks.index(p) if False else p for p in [])
+    # mtask predecessor lists (indices into the topologically ordered array)
+    id2idx = {mt.id: k for k, mt in enumerate(comp.mtasks)}
+    for k, mt in enumerate(comp.mtasks):
+        preds = sorted(id2idx[p] for p in mt.preds)
+        body = ", ".join(f"{x}u" for x in preds) if preds else "0u"
+        out.append(f"static const uint32_t rf_preds_{k}[] = {{{body}}};")
+    out.append("static const rf_mtask rf_mtasks[] = {")
+    for k, mt in enumerate(comp.mtasks):
+        out.append(f'    {{(const void*)&rf_mtask_{k}, "rf_mtask_{k}", {len(mt.preds)}u, rf_preds_{k}, '
+                   f'{mt.cost}u}},')
+    if not comp.mtasks:
+        out.append("    {0}")
+    out.append("};")
+    clk_idx = 0xffffffff
+    for i, p in enumerate(ports):
+        if p["dir"] == 2:
+            clk_idx = i
+            break
+    lay = comp.layout
+    out.append("static const rf_design rf_the_design = {")
+    out.append(f"    RF_DESIGN_ABI, {comp.opts.block}u, \"{d.name}\", 0x{h:x}ull,")
+    out.append(f"    {{{lay.slots['C']}ull, {lay.slots['S']}ull, {lay.slots['I']}ull, {lay.slots['Q']}ull}},")
+    out.append(f"    {lay.lane_major_bytes}ull,")
+    out.append(f"    {len(ports)}u, rf_ports, {len(mems)}u, rf_mems, {len(comp.mtasks)}u, rf_mtasks,")
+    out.append("    (const void*)&rf_k_init, (const void*)&rf_k_eval_end, (const void*)&rf_k_stim,")
+    out.append("    (const void*)&rf_k_fused, (const void*)&rf_k_persistent, (const void*)&rf_k_reset,")
+    out.append(f"    {len(comp.sch.change_vars)}u, {clk_idx}u,")
+    out.append(f"    {man['bytes_seq']}ull, 0ull, {man['bytes_in']}ull, {man['bytes_out']}ull,")
+    out.append(f"    {man['ops_per_cycle']}ull")
+    out.append("};")
+    # the only exported symbol: kernels stay library-local so several design
+    # libraries can live in one process
+    out.append('extern "C" __attribute__((visibility("default"))) const rf_design* '
+               'rf_design_get(void) { return &rf_the_design; }')
+    return "\n".join(out)

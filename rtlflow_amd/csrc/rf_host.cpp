@@ -1,0 +1,161 @@
+// rf_host.cpp -- driver for the one-lane host build of a generated design.
+//
+// The emitter can write every kernel as a host function with the identical
+// body (``void k(rf_args a, uint32_t lane)``); this driver replays them lane
+// by lane with the same protocol the GPU runtime uses. It exists for tests:
+// the survey's "one-lane host build of the emitted model", and a race test
+// that runs the mtasks of each eval in a random topological order (any order
+// the hipGraph may choose must give the same hashes). It is never used on the
+// GPU path.
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <random>
+#include <vector>
+
+#include "rf_design.h"
+
+namespace {
+
+typedef void (*host_kernel)(rf_args, uint32_t);
+
+struct HostModel {
+    const rf_design* d;
+    uint32_t n;
+    uint64_t npad;
+    std::vector<uint8_t> c, mem, done;
+    std::vector<uint16_t> s;
+    std::vector<uint32_t> i, change, cyc, stim, flags;
+    std::vector<uint64_t> q, hash;
+    rf_args a{};
+
+    HostModel(const rf_design* d_, uint32_t n_) : d(d_), n(n_) {
+        npad = ((uint64_t)n + 63) / 64 * 64;
+        c.assign(std::max<uint64_t>(1, d->slots[0]) * npad, 0);
+        s.assign(std::max<uint64_t>(1, d->slots[1]) * npad, 0);
+        i.assign(std::max<uint64_t>(1, d->slots[2]) * npad, 0);
+        q.assign(std::max<uint64_t>(1, d->slots[3]) * npad, 0);
+        mem.assign(std::max<uint64_t>(8, d->mem_row_bytes) * npad, 0);
+        done.assign(npad, 0);
+        change.assign(npad, 1);
+        cyc.assign(npad, 0);
+        stim.assign(npad, 0);
+        flags.assign(RF_FLAG_WORDS, 0);
+        hash.assign(npad, 0xcbf29ce484222325ull);
+        a.c = c.data();
+        a.s = s.data();
+        a.i = i.data();
+        a.q = q.data();
+        a.mem = mem.data();
+        a.change = change.data();
+        a.done = done.data();
+        a.hash = hash.data();
+        a.cyc = cyc.data();
+        a.stim = stim.data();
+        a.flags = flags.data();
+        a.npad = npad;
+        a.nlanes = n;
+        a.converge_limit = 100;
+    }
+    void all(const void* k, uint32_t phase) {
+        rf_args x = a;
+        x.phase = phase;
+        host_kernel f = (host_kernel)k;
+        for (uint32_t l = 0; l < n; ++l) f(x, l);
+    }
+};
+
+// One eval with the host-driven settle loop; ``order`` lists mtask indices in
+// the execution order to use.
+int eval(HostModel& m, const std::vector<uint32_t>& order, uint32_t final_phase) {
+    const rf_design* d = m.d;
+    for (uint32_t it = 0;; ++it) {
+        m.flags[RF_FLAG_CHANGE] = 0;
+        for (uint32_t k : order) m.all(d->mtasks[k].kernel, RF_PHASE_EVAL);
+        if (!d->nchange) break;
+        m.all(d->k_eval_end, RF_PHASE_EVAL);
+        if (!m.flags[RF_FLAG_CHANGE]) break;
+        if (it + 1 >= m.a.converge_limit) return -4;
+    }
+    m.all(d->k_eval_end, final_phase);
+    return 0;
+}
+
+std::vector<uint32_t> topo_order(const rf_design* d, std::mt19937_64* rng) {
+    const uint32_t n = d->nmtasks;
+    std::vector<uint32_t> indeg(n, 0);
+    std::vector<std::vector<uint32_t>> succ(n);
+    for (uint32_t k = 0; k < n; ++k)
+        for (uint32_t j = 0; j < d->mtasks[k].npreds; ++j) {
+            succ[d->mtasks[k].preds[j]].push_back(k);
+            indeg[k]++;
+        }
+    std::vector<uint32_t> ready, out;
+    for (uint32_t k = 0; k < n; ++k)
+        if (!indeg[k]) ready.push_back(k);
+    while (!ready.empty()) {
+        size_t pick = 0;
+        if (rng) pick = (*rng)() % ready.size();
+        const uint32_t k = ready[pick];
+        ready.erase(ready.begin() + pick);
+        out.push_back(k);
+        for (uint32_t s : succ[k])
+            if (--indeg[s] == 0) ready.push_back(s);
+    }
+    return out;
+}
+
+}  // namespace
+
+#define RF_EXPORT __attribute__((visibility("default")))
+
+extern "C" {
+
+// mode 0: mtasks in table order; 1: fused kernel; 2: random topological order
+// per eval (seeded by order_seed).
+RF_EXPORT int rf_host_protocol(const rf_design* d, uint32_t nlanes, uint64_t seed, uint32_t stim0,
+                     uint32_t ncycles, uint32_t mode, uint64_t order_seed, uint64_t* hashes) {
+    HostModel m(d, nlanes);
+    m.a.seed = seed;
+    for (uint32_t l = 0; l < nlanes; ++l) m.stim[l] = stim0 + l;
+    m.all(d->k_init, RF_PHASE_EVAL);
+    std::mt19937_64 rng(order_seed);
+    if (mode == 1) {
+        rf_args x = m.a;
+        x.ncycles = ncycles;
+        host_kernel f = (host_kernel)d->k_fused;
+        for (uint32_t l = 0; l < nlanes; ++l) f(x, l);
+    } else {
+        for (uint32_t c = 0; c < ncycles; ++c) {
+            m.all(d->k_stim, RF_PHASE_EVAL);
+            auto o1 = topo_order(d, mode == 2 ? &rng : nullptr);
+            if (eval(m, o1, RF_PHASE_RISE)) return -4;
+            auto o2 = topo_order(d, mode == 2 ? &rng : nullptr);
+            if (eval(m, o2, RF_PHASE_HASH)) return -4;
+        }
+    }
+    if (m.flags[RF_FLAG_UNCONVERGED]) return -4;
+    memcpy(hashes, m.hash.data(), nlanes * 8ull);
+    return 0;
+}
+
+RF_EXPORT int rf_host_queue(const rf_design* d, uint32_t nlanes, uint64_t seed, const uint32_t* budgets,
+                  uint32_t nstim, const uint32_t* order, uint64_t* out) {
+    HostModel m(d, nlanes);
+    std::vector<uint32_t> ord(nstim);
+    for (uint32_t k = 0; k < nstim; ++k) ord[k] = order ? order[k] : k;
+    m.flags[RF_FLAG_QUEUE] = nlanes;
+    rf_args x = m.a;
+    x.seed = seed;
+    x.budget = budgets;
+    x.order = ord.data();
+    x.out_hash = out;
+    x.nstim = nstim;
+    host_kernel f = (host_kernel)d->k_persistent;
+    for (uint32_t l = 0; l < nlanes; ++l) f(x, l);
+    return m.flags[RF_FLAG_UNCONVERGED] ? -4 : 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+"""Named designs used by the tests, smoke check and benchmark.
+
+``build_all()`` (called by __graft_entry__.build on the CPU) generates,
+compiles and links every entry and records it in ``_build/designs/index.json``;
+``get(name)`` on the GPU box only reads that index and loads the prebuilt
+library -- it never compiles. Each entry also keeps its generator inputs so
+the CPU oracle can be driven with exactly the same design.
+"""
+from __future__ import annotations
+
+import json
+import time
+from pathlib import Path
+from typing import Callable, Dict, Optional
+
+from ..build import BUILD
+
+INDEX = BUILD / "designs" / "index.json"
+
+# config seeds: "RTLF" + config number
+SEED_CFG = {k: 0x52544C4600000000 + k for k in range(1, 6)}
+
+
+class Entry:
+    def __init__(self, name: str, make: Callable[[], dict], opts: dict):
+        self.name = name
+        self.make = make       # -> {"verilog": str, "top": str, "extra": {...}}
+        self.opts = opts
+
+
+def _netlist(**kw):
+    def make():
+        from .netlist import generate
+        src, table = generate(**kw)
+        return {"verilog": src, "top": kw.get("name", "netlist"), "table": table.to_bytes()}
+    return make
+
+
+def _alu(**kw):
+    def make():
+        from .alu import verilog
+        return {"verilog": verilog(), "top": "alu"}
+    return make
+
+
+def _systolic(**kw):
+    def make():
+        from .systolic import verilog
+        return {"verilog": verilog(**kw), "top": "systolic"}
+    return make
+
+
+def _rv32i(**kw):
+    def make():
+        from .rv32i import verilog
+        return {"verilog": verilog(**kw), "top": "rv32i"}
+    return make
+
+
+ENTRIES: Dict[str, Entry] = {}
+
+
+def register(name: str, make, **opts):
+    ENTRIES[name] = Entry(name, make, opts)
+
+
+# small netlists for parity tests
+register("net_tiny", _netlist(n_gates=300, n_ff=30, n_pi=16, n_po=16, depth=8, seed=7,
+                              name="net_tiny"), threads=4)
+register("net_small", _netlist(n_gates=2000, n_ff=200, n_pi=64, n_po=64, depth=20,
+                               seed=SEED_CFG[2] ^ 0xabc, name="net_small"), threads=4)
+# configuration 2: 20,000 gates, 2,000 flip-flops, 64 PI / 64 PO, depth 40
+register("net2", _netlist(seed=SEED_CFG[2], name="net2"), threads=4)
+
+
+def _file_text(path: Path) -> dict:
+    return {}
+
+
+def build_entry(name: str, jobs: int = 8) -> dict:
+    from .. import compile_design
+    e = ENTRIES[name]
+    t0 = time.time()
+    spec = e.make()
+    cd = compile_design(spec["verilog"], top=spec["top"], name=name, jobs=jobs, **e.opts)
+    rec = {"name": name, "lib": str(cd.lib_path.relative_to(BUILD)), "opts": e.opts,
+           "build_s": round(time.time() - t0, 1)}
+    if "table" in spec:
+        tp = cd.lib_path.parent / "gates.bin"
+        tp.write_bytes(spec["table"])
+        rec["table"] = str(tp.relative_to(BUILD))
+    return rec
+
+
+def build_all(names=None, jobs: int = 8, log=print) -> dict:
+    index = load_index()
+    for name in (names or list(ENTRIES)):
+        rec = build_entry(name, jobs=jobs)
+        index[name] = rec
+        if log:
+            log(f"built {name}: {rec['lib']} ({rec['build_s']} s)")
+        INDEX.parent.mkdir(parents=True, exist_ok=True)
+        INDEX.write_text(json.dumps(index, indent=1))
+    return index
+
+
+def load_index() -> dict:
+    if INDEX.exists():
+        return json.loads(INDEX.read_text())
+    return {}
+
+
+def get(name: str):
+    """Prebuilt design -> CompiledDesign (raises if it was not built)."""
+    from .. import CompiledDesign
+    index = load_index()
+    if name not in index:
+        raise RuntimeError(f"design {name!r} is not built; run __graft_entry__.build()")
+    rec = index[name]
+    lib = BUILD / rec["lib"]
+    man = json.loads((lib.parent / "manifest.json").read_text())
+    cd = CompiledDesign(lib, man)
+    cd.record = rec
+    return cd
+
+
+def table_bytes(name: str) -> bytes:
+    rec = load_index()[name]
+    return (BUILD / rec["table"]).read_bytes()

@@ -1,0 +1,75 @@
+"""One-lane host build of a generated design (test utility).
+
+Compiles the emitter's host-function variant of a design (identical kernel
+bodies, one call per lane) with g++ together with csrc/rf_host.cpp, and runs
+the clock-cycle protocol on the CPU. Used by CPU tests to check the emitter
+against the oracle without a GPU, and for the mtask-order race test. It is not
+a fallback for the GPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import os
+import subprocess
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+from .build import BUILD, CSRC, INCLUDE, BuildError
+from .compiler import Compiled, emit_source
+
+
+class HostDesign:
+    def __init__(self, comp: Compiled, opt: str = "-O1"):
+        src = emit_source(comp, host=True)
+        h = hashlib.sha1(src.encode() + opt.encode())
+        for f in ("rf_host.cpp", "rf_ops.h", "rf_stim.h"):
+            h.update((CSRC / f).read_bytes())
+        h.update((INCLUDE / "rf_design.h").read_bytes())
+        key = h.hexdigest()[:16]
+        d = BUILD / "host" / f"{comp.design.name}_{key}"
+        lib = d / f"libH{comp.design.name}.so"
+        if not lib.exists():
+            d.mkdir(parents=True, exist_ok=True)
+            (d / "gen_host.cpp").write_text(src)
+            cmd = ["g++", opt, "-std=c++17", "-fPIC", "-shared", "-w", "-fvisibility=hidden",
+                   "-Wl,-Bsymbolic", "-I", str(INCLUDE), "-I",
+                   str(CSRC), str(d / "gen_host.cpp"), str(CSRC / "rf_host.cpp"), "-o",
+                   str(lib) + ".tmp"]
+            r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+            if r.returncode != 0:
+                raise BuildError("host build failed:\n" + r.stdout[-4000:])
+            os.replace(str(lib) + ".tmp", lib)
+        self.lib = C.CDLL(str(lib), mode=C.RTLD_LOCAL)
+        self.lib.rf_design_get.restype = C.c_void_p
+        self.desc = self.lib.rf_design_get()
+        u32, u64, P = C.c_uint32, C.c_uint64, C.c_void_p
+        self.lib.rf_host_protocol.restype = C.c_int
+        self.lib.rf_host_protocol.argtypes = [P, u32, u64, u32, u32, u32, u64, P]
+        self.lib.rf_host_queue.restype = C.c_int
+        self.lib.rf_host_queue.argtypes = [P, u32, u64, P, u32, P, P]
+        self.comp = comp
+
+    def protocol(self, nlanes: int, seed: int, ncycles: int, mode: int = 0, stim0: int = 0,
+                 order_seed: int = 0) -> np.ndarray:
+        out = np.zeros(nlanes, np.uint64)
+        rc = self.lib.rf_host_protocol(self.desc, nlanes, seed & (2**64 - 1), stim0, ncycles, mode,
+                                       order_seed, out.ctypes.data)
+        if rc:
+            raise RuntimeError(f"host protocol failed: {rc}")
+        return out
+
+    def queue(self, nlanes: int, seed: int, budgets, order=None) -> np.ndarray:
+        b = np.ascontiguousarray(budgets, np.uint32)
+        out = np.zeros(len(b), np.uint64)
+        optr = None
+        if order is not None:
+            o = np.ascontiguousarray(order, np.uint32)
+            optr = o.ctypes.data
+        rc = self.lib.rf_host_queue(self.desc, nlanes, seed & (2**64 - 1), b.ctypes.data, len(b), optr,
+                                    out.ctypes.data)
+        if rc:
+            raise RuntimeError(f"host queue failed: {rc}")
+        return out

@@ -1,0 +1,247 @@
+"""Host-side model API over the C ABI (ctypes).
+
+Mirrors the reference's generated model surface: ``RTLflow(size_t)`` + ``Vtop``
+(src/V3EmitC.cpp:5246-5306, :3510-3792): construct a model for N stimuli, set
+inputs per lane, ``eval()``, read outputs per lane; plus the clock-cycle
+protocol with on-device stimulus/hashing and the batched work queue.
+
+There is no fallback: if the runtime or the design library is missing, or no
+GPU is present, constructing a model raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+from pathlib import Path
+from typing import Dict, List, Optional, Sequence, Union
+
+import numpy as np
+
+from .build import runtime_path
+
+MODE_GRAPH = 0
+MODE_FUSED = 1
+
+_DT = {0: np.uint8, 1: np.uint16, 2: np.uint32, 3: np.uint64}
+
+
+class RFError(RuntimeError):
+    pass
+
+
+class rf_config(C.Structure):
+    _fields_ = [("settle_unroll", C.c_uint32), ("converge_limit", C.c_uint32),
+                ("fused_chunk", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class rf_stats(C.Structure):
+    _fields_ = [("fused_launches", C.c_uint64), ("fused_ms", C.c_double),
+                ("graph_launches", C.c_uint64), ("graph_ms", C.c_double)]
+
+
+_rt = None
+
+
+def runtime() -> C.CDLL:
+    """Load librf_verilated.so and declare its entry points."""
+    global _rt
+    if _rt is not None:
+        return _rt
+    path = runtime_path()
+    if not path.exists():
+        raise RFError(f"runtime library {path} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(str(path), mode=C.RTLD_GLOBAL)
+    P = C.c_void_p
+    u32, u64 = C.c_uint32, C.c_uint64
+    sig = {
+        "rf_abi_version": (u32, []),
+        "rf_create": (C.c_int, [P, u32, C.POINTER(rf_config), C.POINTER(P)]),
+        "rf_destroy": (C.c_int, [P]),
+        "rf_set_port": (C.c_int, [P, u32, u32, u32, P]),
+        "rf_get_port": (C.c_int, [P, u32, u32, u32, P]),
+        "rf_mem_write": (C.c_int, [P, u32, u32, u32, u32, u32, P]),
+        "rf_mem_read": (C.c_int, [P, u32, u32, u32, u32, u32, P]),
+        "rf_set_done": (C.c_int, [P, u32, u32, P]),
+        "rf_eval": (C.c_int, [P]),
+        "rf_begin": (C.c_int, [P, u64, u32]),
+        "rf_run_cycles": (C.c_int, [P, u32, u32]),
+        "rf_hashes": (C.c_int, [P, u32, u32, P]),
+        "rf_submit_queue": (C.c_int, [P, u64, P, u32, P, P]),
+        "rf_set_mem_images": (C.c_int, [P, u32, u32, u32, P]),
+        "rf_lane_state": (C.c_int, [P, u32, P, C.POINTER(C.c_size_t)]),
+        "rf_sync": (C.c_int, [P]),
+        "rf_flags": (C.c_int, [P, P, u32]),
+        "rf_get_stats": (C.c_int, [P, C.POINTER(rf_stats)]),
+        "rf_reset_stats": (C.c_int, [P]),
+        "rf_lanes": (u32, [P]),
+        "rf_lane_stride": (u64, [P]),
+        "rf_last_error": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _rt = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = runtime().rf_last_error().decode(errors="replace")
+        raise RFError(f"rf error {rc}: {msg}")
+
+
+class DesignLib:
+    """A compiled design library (libV<top>.so) plus its manifest."""
+
+    def __init__(self, lib_path: Union[str, Path], manifest: Optional[dict] = None):
+        self.path = Path(lib_path)
+        runtime()
+        # RTLD_LOCAL: several design libraries may be loaded in one process
+        self.lib = C.CDLL(str(self.path), mode=C.RTLD_LOCAL)
+        self.lib.rf_design_get.restype = C.c_void_p
+        self.lib.rf_design_get.argtypes = []
+        self.desc = self.lib.rf_design_get()
+        if manifest is None:
+            mp = self.path.parent / "manifest.json"
+            manifest = json.loads(mp.read_text())
+        self.manifest = manifest
+        self.ports = manifest["ports"]
+        self.port_index = {p["name"]: i for i, p in enumerate(self.ports)}
+        self.mems = manifest.get("mems", [])
+        self.mem_index = {m["name"]: i for i, m in enumerate(self.mems)}
+
+    @property
+    def name(self) -> str:
+        return self.manifest["name"]
+
+
+class Model:
+    """One design instantiated for ``n_lanes`` stimuli on the current GPU."""
+
+    def __init__(self, design: DesignLib, n_lanes: int, settle_unroll: int = 2,
+                 converge_limit: int = 100, fused_chunk: int = 512):
+        self.rt = runtime()
+        self.design = design
+        self.n = int(n_lanes)
+        cfg = rf_config(settle_unroll, converge_limit, fused_chunk, 0)
+        h = C.c_void_p()
+        _check(self.rt.rf_create(design.desc, self.n, C.byref(cfg), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.rt.rf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------ ports
+    def _port(self, name: Union[str, int]):
+        i = self.design.port_index[name] if isinstance(name, str) else int(name)
+        return i, self.design.ports[i]
+
+    def set_port(self, name, values, lane0: int = 0):
+        i, p = self._port(name)
+        arr = np.ascontiguousarray(values, dtype=_DT[p["cls"]])
+        count = arr.shape[0]
+        if arr.size != count * p["words"]:
+            raise RFError(f"port {p['name']} expects {p['words']} words per lane")
+        _check(self.rt.rf_set_port(self.h, i, lane0, count, arr.ctypes.data))
+
+    def get_port(self, name, lane0: int = 0, count: Optional[int] = None) -> np.ndarray:
+        i, p = self._port(name)
+        count = self.n - lane0 if count is None else count
+        out = np.zeros((count, p["words"]), dtype=_DT[p["cls"]])
+        _check(self.rt.rf_get_port(self.h, i, lane0, count, out.ctypes.data))
+        return out[:, 0] if p["words"] == 1 else out
+
+    def mem_write(self, name, values: np.ndarray, lane0: int = 0, elem0: int = 0):
+        mi = self.design.mem_index[name]
+        m = self.design.mems[mi]
+        arr = np.ascontiguousarray(values, dtype=_DT[m["cls"]])
+        count, nelem = arr.shape[0], arr.shape[1]
+        _check(self.rt.rf_mem_write(self.h, mi, lane0, count, elem0, nelem, arr.ctypes.data))
+
+    def mem_read(self, name, lane0: int = 0, count: Optional[int] = None, elem0: int = 0,
+                 nelem: Optional[int] = None) -> np.ndarray:
+        mi = self.design.mem_index[name]
+        m = self.design.mems[mi]
+        count = self.n - lane0 if count is None else count
+        nelem = m["depth"] - elem0 if nelem is None else nelem
+        out = np.zeros((count, nelem, m["words"]), dtype=_DT[m["cls"]])
+        _check(self.rt.rf_mem_read(self.h, mi, lane0, count, elem0, nelem, out.ctypes.data))
+        return out[:, :, 0] if m["words"] == 1 else out
+
+    def set_mem_images(self, name, images: Optional[np.ndarray]):
+        mi = self.design.mem_index[name]
+        m = self.design.mems[mi]
+        if images is None:
+            _check(self.rt.rf_set_mem_images(self.h, mi, 0, 0, None))
+            return
+        arr = np.ascontiguousarray(images, dtype=_DT[m["cls"]])
+        _check(self.rt.rf_set_mem_images(self.h, mi, arr.shape[0], arr.shape[1], arr.ctypes.data))
+
+    def set_done(self, done: np.ndarray, lane0: int = 0):
+        arr = np.ascontiguousarray(done, dtype=np.uint8)
+        _check(self.rt.rf_set_done(self.h, lane0, arr.shape[0], arr.ctypes.data))
+
+    # ------------------------------------------------------- simulation
+    def eval(self):
+        _check(self.rt.rf_eval(self.h))
+
+    def begin(self, seed: int, stim0: int = 0):
+        _check(self.rt.rf_begin(self.h, seed & ((1 << 64) - 1), stim0))
+
+    def run_cycles(self, n: int, mode: int = MODE_FUSED):
+        _check(self.rt.rf_run_cycles(self.h, n, mode))
+
+    def hashes(self) -> np.ndarray:
+        out = np.zeros(self.n, dtype=np.uint64)
+        _check(self.rt.rf_hashes(self.h, 0, self.n, out.ctypes.data))
+        return out
+
+    def submit_queue(self, seed: int, budgets: Sequence[int], order=None) -> np.ndarray:
+        b = np.ascontiguousarray(budgets, dtype=np.uint32)
+        out = np.zeros(len(b), dtype=np.uint64)
+        optr = None
+        if order is not None:
+            o = np.ascontiguousarray(order, dtype=np.uint32)
+            optr = o.ctypes.data
+        _check(self.rt.rf_submit_queue(self.h, seed & ((1 << 64) - 1), b.ctypes.data, len(b), optr,
+                                       out.ctypes.data))
+        return out
+
+    def lane_state(self, lane: int) -> bytes:
+        n = C.c_size_t(0)
+        _check(self.rt.rf_lane_state(self.h, lane, None, C.byref(n)))
+        buf = (C.c_uint8 * n.value)()
+        _check(self.rt.rf_lane_state(self.h, lane, buf, C.byref(n)))
+        return bytes(buf)
+
+    def flags(self) -> np.ndarray:
+        out = np.zeros(8, dtype=np.uint32)
+        _check(self.rt.rf_flags(self.h, out.ctypes.data, 8))
+        return out
+
+    def stats(self) -> dict:
+        s = rf_stats()
+        _check(self.rt.rf_get_stats(self.h, C.byref(s)))
+        return {"fused_launches": s.fused_launches, "fused_ms": s.fused_ms,
+                "graph_launches": s.graph_launches, "graph_ms": s.graph_ms}
+
+    def reset_stats(self):
+        _check(self.rt.rf_reset_stats(self.h))
+
+    def sync(self):
+        _check(self.rt.rf_sync(self.h))

@@ -1,0 +1,88 @@
+"""Pins of the CPU oracle's primitives against published known answers and
+independent restatements (the oracle is only trusted after these pass)."""
+import numpy as np
+import pytest
+
+import oracle
+from rtlflow_amd.designs.netlist import AND, MUX, NAND, OR, XOR, generate
+
+M64 = (1 << 64) - 1
+
+
+def test_fnv1a64_known_answers():
+    # published FNV-1a 64 test vectors (Fowler/Noll/Vo)
+    assert oracle.fnv(b"") == 0xcbf29ce484222325
+    assert oracle.fnv(b"a") == 0xaf63dc4c8601ec8c
+    assert oracle.fnv(b"foobar") == 0x85944171f73967e8
+
+
+def test_splitmix64_known_answers():
+    # SplitMix64 seeded with 0: first outputs of the reference generator
+    assert oracle.splitmix64(0) == 0xe220a8397b1dcdaf
+    assert oracle.splitmix64(0x9E3779B97F4A7C15) == 0x6e789e6aa1b965f4
+
+
+def _py_splitmix(x):
+    z = (x + 0x9E3779B97F4A7C15) & M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def test_stimulus_counter_definition():
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        seed, s, c, k = (int(x) for x in rng.integers(0, 2**63, 4))
+        s %= 1 << 20
+        c %= 1 << 24
+        k %= 256
+        want = _py_splitmix(seed ^ ((s * 0x9E3779B97F4A7C15) & M64) ^ ((c << 8) & M64) ^ k)
+        assert oracle.stim(seed, s, c, k) == want
+
+
+def _py_netlist(table, seed, s, cycles):
+    """Brute-force restatement of the netlist semantics in Python."""
+    n_first = table.n_pi + table.n_ff + table.n_ram_bits
+    v = [0] * (n_first + table.n_gates)
+    q = [0] * table.n_ff
+    h = 0xcbf29ce484222325
+
+    def comb(pi):
+        for i in range(table.n_pi):
+            v[i] = (pi >> i) & 1
+        for i in range(table.n_ff):
+            v[table.n_pi + i] = q[i]
+        for g in range(table.n_gates):
+            a, b, c = (v[int(x)] for x in table.gate_in[g])
+            t = int(table.gate_type[g])
+            v[n_first + g] = {AND: a & b, OR: a | b, XOR: a ^ b, NAND: 1 - (a & b),
+                              MUX: b if a else c}[t]
+
+    for cyc in range(cycles):
+        pi = _py_splitmix(seed ^ ((s * 0x9E3779B97F4A7C15) & M64) ^ (cyc << 8)) & ((1 << table.n_pi) - 1)
+        comb(pi)
+        d = [v[int(x)] for x in table.ff_d]
+        q[:] = d
+        comb(pi)
+        po = sum(v[int(x)] << k for k, x in enumerate(table.po))
+        nb = 1 if len(table.po) <= 8 else 2 if len(table.po) <= 16 else 4 if len(table.po) <= 32 else 8
+        h = oracle.fnv(po.to_bytes(nb, "little"), h)
+    return h
+
+
+def test_netlist_oracle_matches_bruteforce():
+    _, table = generate(n_gates=120, n_ff=12, n_pi=10, n_po=9, depth=5, seed=3)
+    seed = 0x52544C4600000002
+    got = oracle.netlist_run(table.to_bytes(), seed, nstim=5, cycles=7)
+    for s in range(5):
+        assert int(got[s]) == _py_netlist(table, seed, s, 7)
+
+
+def test_netlist_oracle_budgets_and_stim0():
+    _, table = generate(n_gates=80, n_ff=8, n_pi=8, n_po=8, depth=4, seed=9)
+    tb = table.to_bytes()
+    full = oracle.netlist_run(tb, 11, nstim=6, cycles=5)
+    part = oracle.netlist_run(tb, 11, nstim=3, cycles=5, stim0=3)
+    assert (full[3:] == part).all()
+    b = oracle.netlist_run(tb, 11, nstim=6, budgets=[5, 0, 5, 1, 5, 2])
+    assert b[0] == full[0] and b[1] == 0xcbf29ce484222325
