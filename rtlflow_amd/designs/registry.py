@@ -71,6 +71,8 @@ register("net_small", _netlist(n_gates=2000, n_ff=200, n_pi=64, n_po=64, depth=2
                                seed=SEED_CFG[2] ^ 0xabc, name="net_small"), threads=4)
 # configuration 2: 20,000 gates, 2,000 flip-flops, 64 PI / 64 PO, depth 40
 register("net2", _netlist(seed=SEED_CFG[2], name="net2"), threads=4)
+# configuration 1: 4-stage pipelined ALU with a 32 x 32 register file
+register("alu", _alu(), threads=2)
 
 
 def _file_text(path: Path) -> dict:

@@ -86,3 +86,37 @@ def test_netlist_oracle_budgets_and_stim0():
     assert (full[3:] == part).all()
     b = oracle.netlist_run(tb, 11, nstim=6, budgets=[5, 0, 5, 1, 5, 2])
     assert b[0] == full[0] and b[1] == 0xcbf29ce484222325
+
+
+def _py_alu(seed, s, cycles):
+    """Independent restatement of the ALU pipeline in Python."""
+    rf = [0] * 32
+    s1 = [0, 0, 0, 0, 0]
+    s2 = [0, 0, 0, 0]
+    s3 = [0, 0]
+    s4 = 0
+    h = 0xcbf29ce484222325
+    M = 0xffffffff
+    for c in range(cycles):
+        op, rs1, rs2, rd, imm = (_py_splitmix(seed ^ ((s * 0x9E3779B97F4A7C15) & M64) ^ (c << 8) ^ (p << 3))
+                                 for p in range(5))
+        op, rs1, rs2, rd, imm = op & 7, rs1 & 31, rs2 & 31, rd & 31, imm & M
+        o, a, b = s2[0], s2[2], s2[3]
+        res = [(a + b) & M, (a - b) & M, a & b, a | b, a ^ b, (a << (b & 31)) & M, a >> (b & 31),
+               (a * b) & M][o]
+        n_rf = list(rf)
+        n_rf[s3[0]] = s3[1]
+        n_s4 = s3[1]
+        n_s3 = [s2[1], res]
+        n_s2 = [s1[0], s1[3], rf[s1[1]], rf[s1[2]] ^ s1[4]]
+        n_s1 = [op, rs1, rs2, rd, imm]
+        rf, s1, s2, s3, s4 = n_rf, n_s1, n_s2, n_s3, n_s4
+        h = oracle.fnv(s4.to_bytes(4, "little"), h)
+    return h
+
+
+def test_alu_oracle_matches_python_restatement():
+    seed = 0x52544C4600000001
+    got = oracle.alu_run(seed, 4, 300)
+    for s in range(4):
+        assert int(got[s]) == _py_alu(seed, s, 300)
