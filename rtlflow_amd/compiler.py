@@ -307,8 +307,14 @@ def _emit_vertices(comp: Compiled, gen: Gen, vids: List[int], store_in_guard: bo
             gen.ind += 1
             saved = set(gen.avail)
             saved_dirty = set(gen.dirty)
+            # NBA targets read anywhere in this clocked group need a temporary
+            group_reads = set()
+            for p in vx.procs:
+                group_reads |= sch.access[p.id].dep_reads
+            gen.nb_needed = group_reads
             for p in vx.procs:
                 gen.process(p, sch.access[p.id], f"{p.id}")
+            gen.nb_needed = None
             if store_in_guard:
                 written = set()
                 for p in vx.procs:
@@ -414,7 +420,7 @@ def emit_bodies(comp: Compiled) -> List[str]:
     for clk, last in comp.clklast.items():
         gen.ensure(clk)
         gen.declare(last)
-        gen.emit(f"{gen.name(last)} = {gen.name(clk)};")
+        gen.emit(gen.set(last, gen.get(clk)))
         gen.avail.add(last)
         gen.dirty.add(last)
     if comp.sch.change_vars:
@@ -438,7 +444,7 @@ def emit_bodies(comp: Compiled) -> List[str]:
     gen.local = set(d.clocks)
     for clk in d.clocks:
         gen.declare(clk)
-        gen.emit(f"{gen.name(clk)} = v;")
+        gen.emit(gen.set(clk, "v"))
         gen.store(clk)
     out.append(_wrap_body(comp, gen, "rf_body_setclk", extra=", uint32_t v"))
     # output hash
@@ -462,7 +468,7 @@ def emit_bodies(comp: Compiled) -> List[str]:
     for clk, last in comp.clklast.items():
         gen.ensure(clk)
         gen.declare(last)
-        gen.emit(f"{gen.name(last)} = {gen.name(clk)};")
+        gen.emit(gen.set(last, gen.get(clk)))
         gen.avail.add(last)
         gen.dirty.add(last)
     for v in sorted(gen.dirty, key=lambda x: x.id):
@@ -534,6 +540,8 @@ def emit_calls_kernels(comp: Compiled) -> List[str]:
         row = m.depth * (CLASS_BYTES[m.cls] if m.cls != "W" else 4 * m.words)
         lines.append(f"        {{ uint64_t* p = (uint64_t*)(a.mem + (size_t){off} * NP + (size_t)lane * {row}u);")
         lines.append(f"          for (uint32_t k = 0; k < {row // 8}u; ++k) p[k] = 0ull; }}")
+    from .persistent import image_load_lines
+    lines += image_load_lines(comp, "        ")
     lines.append("        rf_body_initial(a, lane);")
     lines += _calls_eval(comp, "        ")
     lines += ["        uint64_t h = RF_FNV_BASIS;",
@@ -564,8 +572,8 @@ def _change_code(comp: Compiled, gen: Gen, result: str):
             gen.emit(f"{result} |= rfw_eq({gen.name(v)}, {gen.name(old)}, {v.width}) ^ 1u;")
             gen.emit(f"rfw_copy({gen.name(old)}, {gen.name(v)}, {v.width});")
         else:
-            gen.emit(f"{result} |= (uint32_t)({gen.name(v)} != {gen.name(old)});")
-            gen.emit(f"{gen.name(old)} = {gen.name(v)};")
+            gen.emit(f"{result} |= (uint32_t)({gen.get(v)} != {gen.get(old)});")
+            gen.emit(gen.set(old, gen.get(v)))
         gen.dirty.add(old)
 
 
@@ -580,7 +588,7 @@ def _hash_code(comp: Compiled, gen: Gen, h: str):
             for w in range(v.words):
                 gen.emit(f"{h} = rf_fnv_bytes({h}, (uint64_t){gen.name(v)}[{w}], 4);")
         else:
-            gen.emit(f"{h} = rf_fnv_bytes({h}, (uint64_t){gen.name(v)}, {CLASS_BYTES[c]});")
+            gen.emit(f"{h} = rf_fnv_bytes({h}, (uint64_t){gen.get(v)}, {CLASS_BYTES[c]});")
 
 
 def _stim_code(comp: Compiled, gen: Gen, sid: str, cyc: str):
@@ -601,8 +609,8 @@ def _stim_code(comp: Compiled, gen: Gen, sid: str, cyc: str):
                     gen.emit("}")
             gen.emit(f"rfw_clean({gen.name(v)}, {v.width});")
         else:
-            gen.emit(f"{gen.name(v)} = ({cty(v.width)})(rf_stim(a.seed, {sid}, {cyc}, {p << 3}u) & "
-                     f"{lit((1 << v.width) - 1, 64)});")
+            gen.emit(gen.set(v, f"({cty(v.width)})(rf_stim(a.seed, {sid}, {cyc}, {p << 3}u) & "
+                                f"{lit((1 << v.width) - 1, 64)})"))
         gen.avail.add(v)
         gen.dirty.add(v)
 
@@ -618,7 +626,7 @@ def emit_eval_end(comp: Compiled) -> str:
     for clk, last in comp.clklast.items():
         gen.ensure(clk)
         gen.declare(last)
-        gen.emit(f"{gen.name(last)} = {gen.name(clk)};")
+        gen.emit(gen.set(last, gen.get(clk)))
         gen.avail.add(last)
         gen.dirty.add(last)
     if comp.sch.change_vars:
@@ -631,7 +639,7 @@ def emit_eval_end(comp: Compiled) -> str:
     gen.ind += 1
     for clk in d.clocks:
         gen.declare(clk)
-        gen.emit(f"{gen.name(clk)} = 1u;")
+        gen.emit(gen.set(clk, "1u"))
         gen.dirty.add(clk)
     gen.ind -= 1
     gen.emit("} else if (a.phase == RF_PHASE_HASH) {")
@@ -658,7 +666,7 @@ def emit_stim(comp: Compiled) -> str:
     _stim_code(comp, gen, "(uint64_t)a.stim[lane]", "(uint64_t)a.cyc[lane]")
     for clk in d.clocks:
         gen.declare(clk)
-        gen.emit(f"{gen.name(clk)} = 0u;")
+        gen.emit(gen.set(clk, "0u"))
         gen.dirty.add(clk)
     if comp.sch.change_vars:
         gen.emit("a.change[lane] = 1u;")
@@ -682,7 +690,7 @@ def _initial_body(comp: Compiled, gen: Gen):
     for clk, last in comp.clklast.items():
         gen.ensure(clk)
         gen.declare(last)
-        gen.emit(f"{gen.name(last)} = {gen.name(clk)};")
+        gen.emit(gen.set(last, gen.get(clk)))
         gen.avail.add(last)
         gen.dirty.add(last)
     comb = _all_comb_vids(comp)
@@ -720,6 +728,7 @@ def emit_init(comp: Compiled) -> str:
     loc |= set(comp.design.clocks) | set(comp.clklast.values()) | set(comp.chglast.values()) | \
         set(sch.change_vars)
     gen.local = loc
+    plan_packing(comp, gen, comp.persistent)
     _initial_body(comp, gen)
     for v in sorted(gen.dirty, key=lambda x: x.id):
         if v in comp.stored:
@@ -738,7 +747,7 @@ def _eval_loop_body(comp: Compiled, gen: Gen):
         gen.ind += 1
     _emit_vertices(comp, gen, all_vids, store_in_guard=False)
     for clk, last in comp.clklast.items():
-        gen.emit(f"{gen.name(last)} = {gen.name(clk)};")
+        gen.emit(gen.set(last, gen.get(clk)))
         gen.dirty.add(last)
     if comp.sch.change_vars:
         _change_code(comp, gen, "ch")
@@ -746,6 +755,64 @@ def _eval_loop_body(comp: Compiled, gen: Gen):
         gen.emit(f"if (it + 1u >= a.converge_limit) {{ {atomic(comp, 'or', '&a.flags[RF_FLAG_UNCONVERGED]', '1u')}; break; }}")
         gen.ind -= 1
         gen.emit("}")
+
+
+def plan_packing(comp: Compiled, gen: Gen, carried: Set[Var]):
+    """Pack the 1-bit register variables of an inline kernel into bit slots of
+    32-bit registers (a 20,000-gate netlist keeps thousands of 1-bit values
+    live, far more than the 512 registers a lane has as whole words).
+
+    ``carried`` variables keep their value across evals (state, clock latches,
+    ports, combinational values read by clocked logic): each gets a slot of
+    its own. Every other 1-bit variable lives only inside one eval; its slot
+    is taken at its defining vertex and freed after its last reader in the
+    eval's topological order (linear-scan allocation)."""
+    sch = comp.sch
+    bits = [v for v in gen.local if v.width == 1]
+    if len(bits) < 64:
+        return
+    pos = {vid: k for k, vid in enumerate(sch.topo)}
+    first_def: Dict[Var, int] = {}
+    last_use: Dict[Var, int] = {}
+    for vid in sch.topo:
+        vx = sch.vertices[vid]
+        k = pos[vid]
+        for v in vx.writes:
+            if v not in first_def:
+                first_def[v] = k
+        for v in vx.reads | vx.need_old:
+            last_use[v] = max(last_use.get(v, -1), k)
+    dedicated = sorted((v for v in bits if v in carried or v not in first_def),
+                       key=lambda x: x.id)
+    transient = [v for v in bits if v not in carried and v in first_def]
+    slot = 0
+    packed: Dict[Var, Tuple[str, int]] = {}
+    for v in dedicated:
+        packed[v] = (f"pk{slot >> 5}", slot & 31)
+        slot += 1
+    base = (slot + 31) & ~31
+    # linear scan over the eval order
+    starts: Dict[int, List[Var]] = {}
+    ends: Dict[int, List[Var]] = {}
+    for v in transient:
+        starts.setdefault(first_def[v], []).append(v)
+        ends.setdefault(max(last_use.get(v, first_def[v]), first_def[v]), []).append(v)
+    free: List[int] = []
+    nxt = base
+    import heapq
+    for k in range(len(sch.topo)):
+        for v in sorted(starts.get(k, []), key=lambda x: x.id):
+            if free:
+                s = heapq.heappop(free)
+            else:
+                s = nxt
+                nxt += 1
+            packed[v] = (f"pk{s >> 5}", s & 31)
+        for v in ends.get(k, []):
+            heapq.heappush(free, packed[v][0] and (int(packed[v][0][2:]) * 32 + packed[v][1]))
+    gen.packed = packed
+    comp.pack_stats = {"dedicated_bits": len(dedicated), "transient_bits": len(transient),
+                       "registers": (nxt + 31) // 32}
 
 
 def _fused_locals(comp: Compiled) -> Set[Var]:
@@ -762,6 +829,7 @@ def emit_fused(comp: Compiled) -> str:
     gen = Gen(comp)
     gen.top_lines = []
     gen.local = _fused_locals(comp)
+    plan_packing(comp, gen, comp.persistent)
     # registers that carry values across evals are loaded once
     for v in sorted(gen.local, key=lambda x: x.id):
         if v in comp.persistent:
@@ -779,7 +847,7 @@ def emit_fused(comp: Compiled) -> str:
     gen.emit("for (uint32_t ph = 0u; ph < 2u; ++ph) {")
     gen.ind += 1
     for clk in d.clocks:
-        gen.emit(f"{gen.name(clk)} = ph;")
+        gen.emit(gen.set(clk, "ph"))
         gen.dirty.add(clk)
     _eval_loop_body(comp, gen)
     gen.ind -= 1

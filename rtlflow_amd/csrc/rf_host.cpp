@@ -21,6 +21,9 @@ namespace {
 
 typedef void (*host_kernel)(rf_args, uint32_t);
 
+uint32_t g_state_lane = ~0u;
+uint8_t* g_state_out = nullptr;
+
 struct HostModel {
     const rf_design* d;
     uint32_t n;
@@ -113,13 +116,42 @@ std::vector<uint32_t> topo_order(const rf_design* d, std::mt19937_64* rng) {
 
 extern "C" {
 
+// Copy per-stimulus memory images (same table layout as rf_set_mem_images:
+// {base pointer, (nstim << 32) | nelem} per memory) into each lane.
+static void load_images(HostModel& m, const uint64_t* images, uint32_t stim0) {
+    if (!images) return;
+    static const size_t es[4] = {1, 2, 4, 8};
+    for (uint32_t j = 0; j < m.d->nmems; ++j) {
+        const rf_mem& md = m.d->mems[j];
+        const uint8_t* src = (const uint8_t*)images[2 * j];
+        const uint32_t ne = (uint32_t)images[2 * j + 1];
+        const uint32_t ns = (uint32_t)(images[2 * j + 1] >> 32);
+        if (!src) continue;
+        const size_t esz = es[md.cls];
+        uint8_t* base[4] = {m.c.data(), (uint8_t*)m.s.data(), (uint8_t*)m.i.data(), (uint8_t*)m.q.data()};
+        for (uint32_t l = 0; l < m.n; ++l) {
+            const uint32_t sid = stim0 + l;
+            if (sid >= ns) continue;
+            const uint8_t* img = src + (size_t)sid * ne * md.words * esz;
+            for (uint32_t k = 0; k < ne * md.words; ++k) {
+                uint8_t* dst = md.layout == RF_MEM_LANE_MAJOR
+                                   ? m.mem.data() + md.loc * m.npad + (size_t)l * md.row_bytes + k * esz
+                                   : base[md.cls] + ((md.loc + k) * m.npad + l) * esz;
+                memcpy(dst, img + k * esz, esz);
+            }
+        }
+    }
+}
+
 // mode 0: mtasks in table order; 1: fused kernel; 2: random topological order
-// per eval (seeded by order_seed).
+// per eval (seeded by order_seed). images: optional per-stimulus memory images.
 RF_EXPORT int rf_host_protocol(const rf_design* d, uint32_t nlanes, uint64_t seed, uint32_t stim0,
-                     uint32_t ncycles, uint32_t mode, uint64_t order_seed, uint64_t* hashes) {
+                               uint32_t ncycles, uint32_t mode, uint64_t order_seed, uint64_t* hashes,
+                               const uint64_t* images) {
     HostModel m(d, nlanes);
     m.a.seed = seed;
     for (uint32_t l = 0; l < nlanes; ++l) m.stim[l] = stim0 + l;
+    load_images(m, images, stim0);
     m.all(d->k_init, RF_PHASE_EVAL);
     std::mt19937_64 rng(order_seed);
     if (mode == 1) {
@@ -138,12 +170,35 @@ RF_EXPORT int rf_host_protocol(const rf_design* d, uint32_t nlanes, uint64_t see
     }
     if (m.flags[RF_FLAG_UNCONVERGED]) return -4;
     memcpy(hashes, m.hash.data(), nlanes * 8ull);
+    if (g_state_lane < nlanes && g_state_out) {
+        // raw state of one lane in the rf_lane_state order: C, S, I, Q slots,
+        // then the lane-major memory region
+        uint8_t* o = g_state_out;
+        for (uint64_t k = 0; k < d->slots[0]; ++k) *o++ = m.c[k * m.npad + g_state_lane];
+        for (uint64_t k = 0; k < d->slots[1]; ++k, o += 2) memcpy(o, &m.s[k * m.npad + g_state_lane], 2);
+        for (uint64_t k = 0; k < d->slots[2]; ++k, o += 4) memcpy(o, &m.i[k * m.npad + g_state_lane], 4);
+        for (uint64_t k = 0; k < d->slots[3]; ++k, o += 8) memcpy(o, &m.q[k * m.npad + g_state_lane], 8);
+        for (uint32_t j = 0; j < d->nmems; ++j) {
+            const rf_mem& md = d->mems[j];
+            if (md.layout != RF_MEM_LANE_MAJOR) continue;
+            memcpy(o + md.loc, m.mem.data() + md.loc * m.npad + (size_t)g_state_lane * md.row_bytes,
+                   md.row_bytes);
+        }
+    }
     return 0;
 }
 
+// Select a lane whose raw state rf_host_protocol copies out after the run.
+RF_EXPORT void rf_host_capture(uint32_t lane, uint8_t* out) {
+    g_state_lane = lane;
+    g_state_out = out;
+}
+
 RF_EXPORT int rf_host_queue(const rf_design* d, uint32_t nlanes, uint64_t seed, const uint32_t* budgets,
-                  uint32_t nstim, const uint32_t* order, uint64_t* out) {
+                            uint32_t nstim, const uint32_t* order, uint64_t* out,
+                            const uint64_t* images) {
     HostModel m(d, nlanes);
+    m.a.images = (const uint8_t*)images;
     std::vector<uint32_t> ord(nstim);
     for (uint32_t k = 0; k < nstim; ++k) ord[k] = order ? order[k] : k;
     m.flags[RF_FLAG_QUEUE] = nlanes;

@@ -81,6 +81,9 @@ RF_HD uint64_t rf_mods64(uint64_t a, uint64_t b, int w) {
 }
 RF_HD uint32_t rf_parity32(uint32_t a) { return (uint32_t)__builtin_popcount(a) & 1u; }
 RF_HD uint32_t rf_parity64(uint64_t a) { return (uint32_t)__builtin_popcountll(a) & 1u; }
+// bit slot b of a packed register of 1-bit values (v must be 0 or 1)
+RF_HD uint32_t rf_bget(uint32_t w, int b) { return (w >> b) & 1u; }
+RF_HD uint32_t rf_bset(uint32_t w, int b, uint32_t v) { return (w & ~(1u << b)) | (v << b); }
 RF_HD uint32_t rf_divu32(uint32_t a, uint32_t b) { return b ? a / b : 0u; }
 RF_HD uint32_t rf_modu32(uint32_t a, uint32_t b) { return b ? a % b : 0u; }
 RF_HD uint64_t rf_divu64(uint64_t a, uint64_t b) { return b ? a / b : 0ull; }

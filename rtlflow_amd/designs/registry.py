@@ -73,6 +73,10 @@ register("net_small", _netlist(n_gates=2000, n_ff=200, n_pi=64, n_po=64, depth=2
 register("net2", _netlist(seed=SEED_CFG[2], name="net2"), threads=4)
 # configuration 1: 4-stage pipelined ALU with a 32 x 32 register file
 register("alu", _alu(), threads=2)
+# configuration 4: 16 x 16 int8 systolic array + 4,096-bit shift/compare chain
+register("systolic", _systolic(), threads=2)
+# configuration 3: 5-stage RV32I pipeline, 64 KB instruction ROM, 16 KB data RAM
+register("rv32i", _rv32i(), threads=2)
 
 
 def _file_text(path: Path) -> dict:

@@ -75,6 +75,13 @@ class Gen:
         self.nb: Dict[Var, str] = {}      # var -> NBA temp name inside a seq process
         self.mem_nb: List[Tuple[str, MemRd]] = []
         self.in_kernel = in_kernel
+        # 1-bit register vars packed into bit slots of 32-bit registers:
+        # var -> (register name, bit)
+        self.packed: Dict[Var, Tuple[str, int]] = {}
+        self.pack_regs: Set[str] = set()
+        # vars whose NBA writes must go through a temporary (read by the
+        # same clocked group); None = every NBA target
+        self.nb_needed: Optional[Set[Var]] = None
 
     # ------------------------------------------------------------ output
     def emit(self, s: str):
@@ -88,6 +95,20 @@ class Gen:
     def name(self, v: Var) -> str:
         return f"v{v.id}"
 
+    def get(self, v: Var) -> str:
+        """Rvalue of a narrow register var."""
+        p = self.packed.get(v)
+        if p is not None:
+            return f"rf_bget({p[0]}, {p[1]})"
+        return self.name(v)
+
+    def set(self, v: Var, expr: str) -> str:
+        """Statement assigning a narrow register var."""
+        p = self.packed.get(v)
+        if p is not None:
+            return f"{p[0]} = rf_bset({p[0]}, {p[1]}, {expr});"
+        return f"{self.name(v)} = {expr};"
+
     def soa(self, v: Var, word: int = 0) -> str:
         loc = self.comp.layout.loc[v]
         c = class_of(v)
@@ -97,6 +118,12 @@ class Gen:
         if v in self.decl:
             return
         self.decl.add(v)
+        p = self.packed.get(v)
+        if p is not None:
+            if p[0] not in self.pack_regs:
+                self.pack_regs.add(p[0])
+                self.emit_top(f"uint32_t {p[0]} = 0u;")
+            return
         if is_wide(v.width):
             self.emit_top(f"uint32_t {self.name(v)}[{v.words}];")
         else:
@@ -112,7 +139,7 @@ class Gen:
             for w in range(v.words):
                 self.emit(f"{n}[{w}] = {self.soa(v, w)};")
         else:
-            self.emit(f"{n} = ({cty(v.width)}){self.soa(v)};")
+            self.emit(self.set(v, f"({cty(v.width)}){self.soa(v)}"))
 
     def store(self, v: Var):
         n = self.name(v)
@@ -121,7 +148,7 @@ class Gen:
             for w in range(v.words):
                 self.emit(f"{self.soa(v, w)} = {n}[{w}];")
         else:
-            self.emit(f"{self.soa(v)} = ({CTY[c]}){n};")
+            self.emit(f"{self.soa(v)} = ({CTY[c]}){self.get(v)};")
 
     def ensure(self, v: Var):
         """Before a read of ``v`` (non-memory)."""
@@ -134,7 +161,7 @@ class Gen:
     # --------------------------------------------------------- var reads
     def read_narrow(self, v: Var) -> str:
         if v in self.local:
-            return self.name(v)
+            return self.get(v)
         return f"(({cty(v.width)}){self.soa(v)})"
 
     def read_wide(self, v: Var) -> str:
@@ -402,14 +429,17 @@ class Gen:
             raise CompileError(f"internal: write to non-register var {v.name}")
         self.declare(v)
         dst = self.target(v)
+        in_nb = v in self.nb
         if lhs.whole:
             if is_wide(v.width):
                 src = self.exw(s.rhs)
                 self.emit(f"rfw_copy({dst}, {src}, {v.width});")
-            else:
+            elif in_nb:
                 self.emit(f"{dst} = {self.ex(s.rhs)};")
+            else:
+                self.emit(self.set(v, self.ex(s.rhs)))
         else:
-            if v not in self.avail and not (v in self.nb):
+            if v not in self.avail and not in_nb:
                 self.ensure(v)
             lsb = str(lhs.lsb) if isinstance(lhs.lsb, int) else self.amount(lhs.lsb)
             if is_wide(v.width):
@@ -419,10 +449,15 @@ class Gen:
                     self.emit(f"rfw_put64({dst}, {v.width}, {lsb}, {lhs.width}, "
                               f"(uint64_t)({self.ex(s.rhs)}));")
             else:
-                self.emit(f"{dst} = ({cty(v.width)})rf_ins64((uint64_t){dst}, {v.width}, {lsb}, "
-                          f"{lhs.width}, (uint64_t)({self.ex(s.rhs)}));")
-        if v not in self.nb:
+                cur = dst if in_nb else self.get(v)
+                val = (f"({cty(v.width)})rf_ins64((uint64_t){cur}, {v.width}, {lsb}, "
+                       f"{lhs.width}, (uint64_t)({self.ex(s.rhs)}))")
+                self.emit(f"{dst} = {val};" if in_nb else self.set(v, val))
+        if not in_nb:
             self.dirty.add(v)
+            # the register now holds the variable's current value on this path
+            # (a later partial write must not reload the stale stored copy)
+            self.avail.add(v)
 
     def mem_assign(self, s: Assign):
         lhs = s.lhs
@@ -576,19 +611,24 @@ class Gen:
         """Emit one process body (blocking/NBA semantics)."""
         self.uid = uid
         if p.kind == "seq":
-            nbv = [v for v in sorted(acc.nb_writes, key=lambda x: x.id) if not v.is_mem]
+            # old values of conditionally / partially written targets
+            for v in sorted(acc.need_old, key=lambda x: x.id):
+                if not v.is_mem:
+                    self.ensure(v)
+            # a non-blocking target needs a temporary only if the clocked
+            # group reads it (the read must see the value before the edge)
+            nbv = [v for v in sorted(acc.nb_writes, key=lambda x: x.id)
+                   if not v.is_mem and (self.nb_needed is None or v in self.nb_needed)]
             for v in nbv:
                 self.declare(v)
                 t = f"nb{v.id}_{uid}"
                 if is_wide(v.width):
                     self.emit(f"uint32_t {t}[{v.words}];")
                     if v in acc.need_old:
-                        self.ensure(v)
                         self.emit(f"rfw_copy({t}, {self.name(v)}, {v.width});")
                 else:
                     if v in acc.need_old:
-                        self.ensure(v)
-                        self.emit(f"{cty(v.width)} {t} = {self.name(v)};")
+                        self.emit(f"{cty(v.width)} {t} = {self.get(v)};")
                     else:
                         self.emit(f"{cty(v.width)} {t} = 0;")
                 self.nb[v] = t
@@ -598,7 +638,7 @@ class Gen:
                 if is_wide(v.width):
                     self.emit(f"rfw_copy({self.name(v)}, {t}, {v.width});")
                 else:
-                    self.emit(f"{self.name(v)} = {t};")
+                    self.emit(self.set(v, t))
                 self.dirty.add(v)
             self.commit_mem_nb()
         else:

@@ -47,29 +47,62 @@ class HostDesign:
         self.desc = self.lib.rf_design_get()
         u32, u64, P = C.c_uint32, C.c_uint64, C.c_void_p
         self.lib.rf_host_protocol.restype = C.c_int
-        self.lib.rf_host_protocol.argtypes = [P, u32, u64, u32, u32, u32, u64, P]
+        self.lib.rf_host_protocol.argtypes = [P, u32, u64, u32, u32, u32, u64, P, P]
         self.lib.rf_host_queue.restype = C.c_int
-        self.lib.rf_host_queue.argtypes = [P, u32, u64, P, u32, P, P]
+        self.lib.rf_host_queue.argtypes = [P, u32, u64, P, u32, P, P, P]
         self.comp = comp
+        self.mems = comp.mem_table()
+
+    def _images(self, images):
+        """{memory name: array [nstim, nelem]} -> (table, keep-alive list)."""
+        if not images:
+            return None, []
+        dt = {0: np.uint8, 1: np.uint16, 2: np.uint32, 3: np.uint64}
+        table = np.zeros(2 * max(1, len(self.mems)), np.uint64)
+        keep = []
+        for j, m in enumerate(self.mems):
+            if m["name"] in images:
+                arr = np.ascontiguousarray(images[m["name"]], dtype=dt[m["cls"]])
+                keep.append(arr)
+                table[2 * j] = arr.ctypes.data
+                table[2 * j + 1] = (arr.shape[0] << 32) | arr.shape[1]
+        keep.append(table)
+        return table, keep
 
     def protocol(self, nlanes: int, seed: int, ncycles: int, mode: int = 0, stim0: int = 0,
-                 order_seed: int = 0) -> np.ndarray:
+                 order_seed: int = 0, images=None) -> np.ndarray:
         out = np.zeros(nlanes, np.uint64)
+        table, keep = self._images(images)
         rc = self.lib.rf_host_protocol(self.desc, nlanes, seed & (2**64 - 1), stim0, ncycles, mode,
-                                       order_seed, out.ctypes.data)
+                                       order_seed, out.ctypes.data,
+                                       None if table is None else table.ctypes.data)
         if rc:
             raise RuntimeError(f"host protocol failed: {rc}")
         return out
 
-    def queue(self, nlanes: int, seed: int, budgets, order=None) -> np.ndarray:
+    def lane_state(self, lane: int, nlanes: int, seed: int, ncycles: int, mode: int = 0,
+                   images=None) -> dict:
+        """Run the protocol and return one lane's stored variables by name."""
+        from .statediff import decode_state, state_bytes
+        buf = np.zeros(state_bytes(self.comp.manifest()), np.uint8)
+        self.lib.rf_host_capture.argtypes = [C.c_uint32, C.c_void_p]
+        self.lib.rf_host_capture(lane, buf.ctypes.data)
+        try:
+            self.protocol(nlanes, seed, ncycles, mode=mode, images=images)
+        finally:
+            self.lib.rf_host_capture(0xffffffff, None)
+        return decode_state(self.comp.manifest(), buf.tobytes())
+
+    def queue(self, nlanes: int, seed: int, budgets, order=None, images=None) -> np.ndarray:
         b = np.ascontiguousarray(budgets, np.uint32)
         out = np.zeros(len(b), np.uint64)
         optr = None
         if order is not None:
             o = np.ascontiguousarray(order, np.uint32)
             optr = o.ctypes.data
+        table, keep = self._images(images)
         rc = self.lib.rf_host_queue(self.desc, nlanes, seed & (2**64 - 1), b.ctypes.data, len(b), optr,
-                                    out.ctypes.data)
+                                    out.ctypes.data, None if table is None else table.ctypes.data)
         if rc:
             raise RuntimeError(f"host queue failed: {rc}")
         return out

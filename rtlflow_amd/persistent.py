@@ -16,10 +16,44 @@ rule, which bounds the tail by the longest single stimulus).
 from __future__ import annotations
 
 from .compiler import (_eval_loop_body, _fused_locals, _hash_code, _initial_body, _prologue,
-                       _stim_code, atomic, kernel_header)
+                       _stim_code, atomic, kernel_header, plan_packing)
 from .emit_hip import Gen, is_wide
 from .ir import CLASS_BYTES
 from .layout import class_of
+
+
+def image_load_lines(comp, ind: str) -> list:
+    """Copy the stimulus's memory images (rf_set_mem_images) into the lane.
+    a.images is a table of {image base pointer, (nstim << 32) | nelem} per
+    memory of the design's memory table."""
+    d = comp.design
+    lay = comp.layout
+    mems = [v for v in d.vars if v.is_mem and lay.has(v)]
+    if not mems:
+        return []
+    cty_ = {"C": "uint8_t", "S": "uint16_t", "I": "uint32_t", "Q": "uint64_t"}
+    arr = {"C": "a.c", "S": "a.s", "I": "a.i", "Q": "a.q"}
+    out = [f"{ind}if (a.images) {{",
+           f"{ind}    const uint64_t* im = (const uint64_t*)a.images;"]
+    for j, m in enumerate(mems):
+        c = class_of(m)
+        T = cty_[c]
+        n = m.words
+        out.append(f"{ind}    {{ const {T}* src = (const {T}*)im[{2 * j}];")
+        out.append(f"{ind}      const uint32_t ne = (uint32_t)im[{2 * j + 1}];")
+        out.append(f"{ind}      if (src && sid < (uint32_t)(im[{2 * j + 1}] >> 32)) {{")
+        out.append(f"{ind}        src += (size_t)sid * ne * {n}u;")
+        if m in lay.lane_major:
+            row = m.depth * (CLASS_BYTES[m.cls] if m.cls != "W" else 4 * m.words)
+            out.append(f"{ind}        {T}* dst = ({T}*)(a.mem + (size_t){lay.lane_major[m]} * a.npad"
+                       f" + (size_t)lane * {row}u);")
+            out.append(f"{ind}        for (uint32_t k = 0; k < ne * {n}u; ++k) dst[k] = src[k];")
+        else:
+            out.append(f"{ind}        for (uint32_t k = 0; k < ne * {n}u; ++k) "
+                       f"{arr[c]}[((size_t){lay.loc[m]} + k) * a.npad + lane] = src[k];")
+        out.append(f"{ind}      }} }}")
+    out.append(f"{ind}}}")
+    return out
 
 
 def emit_persistent(comp) -> str:
@@ -27,6 +61,7 @@ def emit_persistent(comp) -> str:
     gen = Gen(comp)
     gen.top_lines = []
     gen.local = _fused_locals(comp)
+    plan_packing(comp, gen, comp.persistent)
     for v in sorted(gen.local, key=lambda x: x.id):
         gen.declare(v)
     lay = comp.layout
@@ -39,8 +74,10 @@ def emit_persistent(comp) -> str:
     for v in sorted(gen.local, key=lambda x: x.id):
         if is_wide(v.width):
             gen.emit(f"rfw_zero({gen.name(v)}, {v.width});")
-        else:
+        elif v not in gen.packed:
             gen.emit(f"{gen.name(v)} = 0;")
+    for r in sorted(gen.pack_regs):
+        gen.emit(f"{r} = 0u;")
     gen.avail = set(gen.local)
     for m in [v for v in d.vars if v.is_mem and lay.has(v)]:
         if m in lay.lane_major:
@@ -50,6 +87,8 @@ def emit_persistent(comp) -> str:
             c = class_of(m)
             arr = {"C": "sC", "S": "sS", "I": "sI", "Q": "sQ"}[c]
             gen.emit(f"for (uint32_t k = 0; k < {m.slots}u; ++k) {arr}[((size_t){lay.loc[m]} + k) * NP] = 0;")
+    for ln in image_load_lines(comp, ""):
+        gen.emit(ln)
     _initial_body(comp, gen)
     gen.emit("uint64_t h = RF_FNV_BASIS;")
     gen.emit("for (uint32_t cyc = 0u; cyc < budget; ++cyc) {")
@@ -59,7 +98,7 @@ def emit_persistent(comp) -> str:
     gen.emit("for (uint32_t ph = 0u; ph < 2u; ++ph) {")
     gen.ind += 1
     for clk in d.clocks:
-        gen.emit(f"{gen.name(clk)} = ph;")
+        gen.emit(gen.set(clk, "ph"))
         gen.dirty.add(clk)
     _eval_loop_body(comp, gen)
     gen.ind -= 1

@@ -58,3 +58,41 @@ def test_two_design_libraries_in_one_process():
         libs.append((HostDesign(Compiled(elaborate(src), Options(threads=2))), table.to_bytes()))
     for hd, tb in libs:
         assert (hd.protocol(4, seed, 9) == oracle.netlist_run(tb, seed, nstim=4, cycles=9)).all()
+
+
+@pytest.mark.parametrize("threads", [1, 3])
+def test_alu_host_build(threads):
+    from rtlflow_amd.designs.alu import verilog
+    seed = 0x52544C4600000001
+    ref = oracle.alu_run(seed, 6, 150)
+    hd = HostDesign(Compiled(elaborate(verilog()), Options(threads=threads)))
+    for mode in (0, 1, 2):
+        assert (hd.protocol(6, seed, 150, mode=mode, order_seed=7) == ref).all()
+    budgets = [150, 0, 3, 77, 150, 1]
+    want = [int(oracle.alu_run(seed, 1, b, stim0=i)[0]) for i, b in enumerate(budgets)]
+    assert list(hd.queue(4, seed, budgets)) == want
+
+
+def test_systolic_host_build():
+    from rtlflow_amd.designs.systolic import verilog
+    seed = 0x52544C4600000004
+    ref = oracle.systolic_run(seed, 3, 530)
+    hd = HostDesign(Compiled(elaborate(verilog()), Options(threads=2)))
+    for mode in (0, 1):
+        assert (hd.protocol(3, seed, 530, mode=mode) == ref).all()
+
+
+def test_rv32i_host_build_and_iss():
+    import ctypes as C
+    import numpy as np
+    from rtlflow_amd.designs import rv32i
+    seed = 0x52544C4600000003
+    n = 4
+    progs = rv32i.programs(seed, np.arange(n))
+    ref = oracle.rv32i_run(progs, seed, [2500] * n)
+    for threads in (1, 2):
+        hd = HostDesign(Compiled(elaborate(rv32i.verilog()), Options(threads=threads)))
+        for mode in (0, 1, 2):
+            assert (hd.protocol(n, seed, 2500, mode=mode, order_seed=9, images={"imem": progs}) == ref).all()
+        b = [2500, 7, 0, 1200]
+        assert (hd.queue(2, seed, b, images={"imem": progs}) == oracle.rv32i_run(progs, seed, b)).all()

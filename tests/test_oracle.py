@@ -120,3 +120,74 @@ def test_alu_oracle_matches_python_restatement():
     got = oracle.alu_run(seed, 4, 300)
     for s in range(4):
         assert int(got[s]) == _py_alu(seed, s, 300)
+
+
+def _py_systolic(seed, s, cycles, n=16, chain_bits=4096):
+    """Independent restatement of the systolic array + chain with Python ints."""
+    a = [[0] * n for _ in range(n)]
+    b = [[0] * n for _ in range(n)]
+    acc = [[0] * n for _ in range(n)]
+    chain = 0
+    cmp_ = par = 0
+    h = 0xcbf29ce484222325
+    sx = lambda v: v - 256 if v >= 128 else v
+    half = chain_bits // 2
+    for c in range(cycles):
+        r = [_py_splitmix(seed ^ ((s * 0x9E3779B97F4A7C15) & M64) ^ (c << 8) ^ k) for k in (0, 1, 8, 9)]
+        row = (r[0] | (r[1] << 64))
+        col = (r[2] | (r[3] << 64))
+        rowb = [(row >> (8 * i)) & 0xff for i in range(n)]
+        colb = [(col >> (8 * j)) & 0xff for j in range(n)]
+        na = [[rowb[i] if j == 0 else a[i][j - 1] for j in range(n)] for i in range(n)]
+        nb = [[colb[j] if i == 0 else b[i - 1][j] for j in range(n)] for i in range(n)]
+        nacc = [[(acc[i][j] + sx(a[i][j]) * sx(b[i][j])) & 0xffffffff for j in range(n)] for i in range(n)]
+        ncmp = int((chain >> half) < (chain & ((1 << half) - 1)))
+        npar = bin(chain).count("1") & 1
+        chain = ((chain << 8) | (rowb[0] ^ colb[0])) & ((1 << chain_bits) - 1)
+        a, b, acc, cmp_, par = na, nb, nacc, ncmp, npar
+        x = 0
+        for i in range(n):
+            for j in range(n):
+                x ^= acc[i][j]
+        h = oracle.fnv(x.to_bytes(4, "little"), h)
+        h = oracle.fnv(bytes([(cmp_ << 1) | par]), h)
+    return h
+
+
+def test_systolic_oracle_matches_python_restatement():
+    seed = 0x52544C4600000004
+    got = oracle.systolic_run(seed, 2, 600)
+    for s in range(2):
+        assert int(got[s]) == _py_systolic(seed, s, 600)
+
+
+def test_rv32i_pipeline_model_matches_iss():
+    """The cycle-accurate pipeline model must reach the architectural state
+    the instruction-set simulator computes (forwarding, stalls, flushes)."""
+    import ctypes as C
+    from rtlflow_amd.designs import rv32i
+    L = oracle.lib()
+    L.or_rv32i_state.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]
+    L.or_rv32i_iss.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]
+    progs = rv32i.programs(0x52544C4600000003, np.arange(24))
+    for p in progs:
+        p = np.ascontiguousarray(p)
+        rf1, rf2 = np.zeros(32, np.uint32), np.zeros(32, np.uint32)
+        dm1, dm2 = np.zeros(4096, np.uint32), np.zeros(4096, np.uint32)
+        retired = L.or_rv32i_iss(p.ctypes.data, 4096, 4095 * 4, 100000, rf1.ctypes.data, dm1.ctypes.data)
+        assert 1000 < retired < 4096
+        L.or_rv32i_state(p.ctypes.data, 4096, 3 * 4096 + 50, rf2.ctypes.data, dm2.ctypes.data)
+        assert (rf1 == rf2).all() and (dm1 == dm2).all()
+        assert (rf1 != 0).sum() > 4 and (dm1 != 0).sum() > 10
+
+
+def test_rv32i_programs_are_valid():
+    from rtlflow_amd.designs import rv32i
+    p = rv32i.programs(7, np.arange(8))
+    assert p.shape == (8, 4096) and (p[:, -1] == 0x63).all()
+    ops = p & 0x7f
+    assert set(np.unique(ops)) <= {0x33, 0x13, 0x37, 0x17, 0x03, 0x23, 0x63}
+    br = p[ops == 0x63]
+    assert len(br) > 100
+    b = rv32i.budgets(1, 20000)
+    assert b.min() >= 1000 and b.max() <= 100000 and 4500 < np.median(b) < 5500
