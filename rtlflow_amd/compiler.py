@@ -16,7 +16,7 @@ from .emit_hip import CTY, Gen, cty, edge_cond, is_wide, lit
 from .instrcount import process_cost
 from .ir import CLASS_BYTES, Design, Ref, Var, words_of
 from .layout import Layout, class_of
-from .order import CompileError, Schedule, build_schedule
+from .order import CompileError, Schedule, build_schedule, split_seq_processes
 from .partition import MTask, partition, partition_stats
 
 BLOCK = 256
@@ -27,7 +27,7 @@ class Options:
     def __init__(self, threads: int = 8, block: int = BLOCK, lane_major_min_depth: int = 64,
                  cost_fn: Optional[Callable] = None, settle_unroll: int = 2,
                  converge_limit: int = 100, state_in_regs: bool = True, keep=None,
-                 mtask_stats: bool = False, inline_limit: int = 6000):
+                 mtask_stats: bool = False, inline_limit: int = 6000, split_always: bool = True):
         self.threads = threads            # --threads: partition granularity (cpLimit)
         self.block = block
         self.lane_major_min_depth = lane_major_min_depth
@@ -38,6 +38,8 @@ class Options:
         self.keep = keep or set()
         # designs whose summed logic cost exceeds this use the "calls" form
         self.inline_limit = inline_limit
+        # split clocked blocks into independent statement groups
+        self.split_always = split_always
 
     def key(self) -> str:
         return f"t{self.threads}_b{self.block}_lm{self.lane_major_min_depth}"
@@ -50,6 +52,8 @@ class Compiled:
         self.design = design
         self.opts = opts or Options()
         d = design
+        if self.opts.split_always:
+            split_seq_processes(d)
         # synthetic clock-latch variables (__Vclklast__, src/V3Clock.cpp:64-92)
         self.clklast: Dict[Var, Var] = {}
         for clk in d.clocks:
@@ -312,8 +316,7 @@ def _emit_vertices(comp: Compiled, gen: Gen, vids: List[int], store_in_guard: bo
             for p in vx.procs:
                 group_reads |= sch.access[p.id].dep_reads
             gen.nb_needed = group_reads
-            for p in vx.procs:
-                gen.process(p, sch.access[p.id], f"{p.id}")
+            gen.seq_group(vx.procs, [sch.access[p.id] for p in vx.procs], f"{vx.procs[0].id}")
             gen.nb_needed = None
             if store_in_guard:
                 written = set()
@@ -337,6 +340,7 @@ def _mtask_gen(comp: Compiled, k: int) -> Gen:
     mt = comp.mtasks[k]
     gen = Gen(comp)
     gen.top_lines = []
+    gen.write_through = True
     gen.local = _vertex_vars(comp, mt.vids)
     # clocks and clock latches used by edge guards
     for vid in mt.vids:

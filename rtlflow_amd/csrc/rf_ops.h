@@ -308,6 +308,72 @@ RF_HD void rfw_extract(uint32_t* o, int wr, const uint32_t* a, int wa, uint64_t 
     for (int k = 0; k < rfw_words(wr); ++k) o[k] = (uint32_t)rfw_get64(a, wa, lsb + 32ull * k, 32);
     rfw_clean(o, wr);
 }
+// Wide division (restoring, one quotient bit per step), widths up to
+// RFW_DIV_MAX_WORDS * 32 bits; x / 0 = x % 0 = 0 (VL_DIV_WWW / VL_MODDIV_WWW,
+// include/rf_verilated.h:1856-1861). q or r may be null.
+#define RFW_DIV_MAX_WORDS 16
+RF_HD void rfw_divmodu(uint32_t* q, uint32_t* r, const uint32_t* a, const uint32_t* b, int w) {
+    const int n = rfw_words(w);
+    uint32_t rem[RFW_DIV_MAX_WORDS + 1];
+    uint32_t quo[RFW_DIV_MAX_WORDS];
+    for (int i = 0; i <= n; ++i) rem[i] = 0;
+    for (int i = 0; i < n; ++i) quo[i] = 0;
+    uint32_t nz = 0;
+    for (int i = 0; i < n; ++i) nz |= b[i];
+    if (nz) {
+        for (int bit = w - 1; bit >= 0; --bit) {
+            // rem = (rem << 1) | a[bit]
+            for (int i = n; i > 0; --i) rem[i] = (rem[i] << 1) | (rem[i - 1] >> 31);
+            rem[0] = (rem[0] << 1) | rfw_bit(a, bit);
+            // rem >= b ?
+            int ge = 1;
+            for (int i = n; i >= 0; --i) {
+                const uint32_t bi = i < n ? b[i] : 0u;
+                if (rem[i] != bi) { ge = rem[i] > bi; break; }
+            }
+            if (ge) {
+                uint64_t borrow = 0;
+                for (int i = 0; i <= n; ++i) {
+                    const uint64_t bi = i < n ? b[i] : 0u;
+                    const uint64_t t = (uint64_t)rem[i] - bi - borrow;
+                    rem[i] = (uint32_t)t;
+                    borrow = (t >> 63) & 1u;
+                }
+                quo[bit >> 5] |= 1u << (bit & 31);
+            }
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        if (q) q[i] = quo[i];
+        if (r) r[i] = nz ? rem[i] : 0u;
+    }
+}
+RF_HD void rfw_divu(uint32_t* o, const uint32_t* a, const uint32_t* b, int w) {
+    rfw_divmodu(o, nullptr, a, b, w);
+}
+RF_HD void rfw_modu(uint32_t* o, const uint32_t* a, const uint32_t* b, int w) {
+    rfw_divmodu(nullptr, o, a, b, w);
+}
+// signed: truncating division, remainder takes the dividend's sign
+RF_HD void rfw_divmods(uint32_t* o, const uint32_t* a, const uint32_t* b, int w, int want_rem) {
+    uint32_t ua[RFW_DIV_MAX_WORDS], ub[RFW_DIV_MAX_WORDS], res[RFW_DIV_MAX_WORDS];
+    const uint32_t sa = rfw_bit(a, w - 1), sb = rfw_bit(b, w - 1);
+    if (sa) rfw_neg(ua, a, w); else rfw_copy(ua, a, w);
+    if (sb) rfw_neg(ub, b, w); else rfw_copy(ub, b, w);
+    if (want_rem) {
+        rfw_divmodu(nullptr, res, ua, ub, w);
+        if (sa) rfw_neg(o, res, w); else rfw_copy(o, res, w);
+    } else {
+        rfw_divmodu(res, nullptr, ua, ub, w);
+        if (sa != sb) rfw_neg(o, res, w); else rfw_copy(o, res, w);
+    }
+}
+RF_HD void rfw_divs(uint32_t* o, const uint32_t* a, const uint32_t* b, int w) {
+    rfw_divmods(o, a, b, w, 0);
+}
+RF_HD void rfw_mods(uint32_t* o, const uint32_t* a, const uint32_t* b, int w) {
+    rfw_divmods(o, a, b, w, 1);
+}
 // shift amount given as a wide value: clamp to a 64-bit amount (>= 2^32 is
 // "at least the width" for every width this library supports)
 RF_HD uint64_t rfw_amount(const uint32_t* a, int w) {

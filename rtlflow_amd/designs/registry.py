@@ -107,7 +107,20 @@ def build_all(names=None, jobs: int = 8, log=print) -> dict:
             log(f"built {name}: {rec['lib']} ({rec['build_s']} s)")
         INDEX.parent.mkdir(parents=True, exist_ok=True)
         INDEX.write_text(json.dumps(index, indent=1))
+        _prune(name, Path(rec["lib"]).parent.name)
     return index
+
+
+def _prune(name: str, keep_dir: str):
+    """Remove older builds of one design (they would otherwise travel with the
+    tree to every GPU run)."""
+    import shutil
+    root = BUILD / "designs"
+    for d in root.glob(f"{name}_*"):
+        suffix = d.name[len(name) + 1:]
+        if d.is_dir() and d.name != keep_dir and len(suffix) == 16 and \
+                all(c in "0123456789abcdef" for c in suffix):
+            shutil.rmtree(d)
 
 
 def load_index() -> dict:
@@ -133,3 +146,15 @@ def get(name: str):
 def table_bytes(name: str) -> bytes:
     rec = load_index()[name]
     return (BUILD / rec["table"]).read_bytes()
+
+
+def _fuzz(seed: int):
+    def make():
+        from .fuzz import verilog
+        return {"verilog": verilog(seed, name=f"fuzz{seed}"), "top": f"fuzz{seed}"}
+    return make
+
+
+# random operator-coverage designs (compared with the IR interpreter)
+for _s in (1, 2, 3, 4):
+    register(f"fuzz{_s}", _fuzz(_s), threads=3)

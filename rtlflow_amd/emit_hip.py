@@ -82,6 +82,7 @@ class Gen:
         # vars whose NBA writes must go through a temporary (read by the
         # same clocked group); None = every NBA target
         self.nb_needed: Optional[Set[Var]] = None
+        self.write_through = False
 
     # ------------------------------------------------------------ output
     def emit(self, s: str):
@@ -413,6 +414,13 @@ class Gen:
                 b = self.exw(args[1])
                 self.emit(f"rfw_{op}({t}, {a}, {b}, {w});")
                 return t
+            if op in ("divu", "modu", "divs", "mods"):
+                if words_of(w) > 16:
+                    raise CompileError(f"division wider than 512 bits is not supported ({w})")
+                a = self.exw(args[0])
+                b = self.exw(args[1])
+                self.emit(f"rfw_{op}({t}, {a}, {b}, {w});")
+                return t
         raise CompileError(f"wide expression {e!r} not supported")
 
     # ------------------------------------------------------- statements
@@ -458,6 +466,15 @@ class Gen:
             # the register now holds the variable's current value on this path
             # (a later partial write must not reload the stale stored copy)
             self.avail.add(v)
+            self._write_through(v)
+
+    def _write_through(self, v: Var):
+        """In per-mtask kernels, store a stored variable as soon as it is
+        written: its register can then be released instead of staying live
+        until the kernel's end (thousands of values in a large mtask)."""
+        if self.write_through and v in self.comp.stored:
+            self.store(v)
+            self.dirty.discard(v)
 
     def mem_assign(self, s: Assign):
         lhs = s.lhs
@@ -607,40 +624,55 @@ class Gen:
         self.avail = res
 
     # ------------------------------------------------------- processes
+    def seq_group(self, procs, accs, uid: str):
+        """Emit a group of clocked processes that share one edge event: every
+        statement reads the values from before the edge, so non-blocking
+        writes (to variables and memories) are committed only after the last
+        process of the group."""
+        self.uid = uid
+        need_old = set()
+        nb_writes = set()
+        for acc in accs:
+            need_old |= acc.need_old
+            nb_writes |= acc.nb_writes
+        # old values of conditionally / partially written targets
+        for v in sorted(need_old, key=lambda x: x.id):
+            if not v.is_mem:
+                self.ensure(v)
+        # a non-blocking target needs a temporary only if the group reads it
+        nbv = [v for v in sorted(nb_writes, key=lambda x: x.id)
+               if not v.is_mem and (self.nb_needed is None or v in self.nb_needed)]
+        for v in nbv:
+            self.declare(v)
+            t = f"nb{v.id}_{uid}"
+            if is_wide(v.width):
+                self.emit(f"uint32_t {t}[{v.words}];")
+                if v in need_old:
+                    self.emit(f"rfw_copy({t}, {self.name(v)}, {v.width});")
+            else:
+                if v in need_old:
+                    self.emit(f"{cty(v.width)} {t} = {self.get(v)};")
+                else:
+                    self.emit(f"{cty(v.width)} {t} = 0;")
+            self.nb[v] = t
+        for k, p in enumerate(procs):
+            self.uid = f"{uid}_{k}"
+            self.stmts(p.stmts)
+        for v in nbv:
+            t = self.nb.pop(v)
+            if is_wide(v.width):
+                self.emit(f"rfw_copy({self.name(v)}, {t}, {v.width});")
+            else:
+                self.emit(self.set(v, t))
+            self.dirty.add(v)
+            self._write_through(v)
+        self.commit_mem_nb()
+
     def process(self, p: Process, acc, uid: str):
         """Emit one process body (blocking/NBA semantics)."""
         self.uid = uid
         if p.kind == "seq":
-            # old values of conditionally / partially written targets
-            for v in sorted(acc.need_old, key=lambda x: x.id):
-                if not v.is_mem:
-                    self.ensure(v)
-            # a non-blocking target needs a temporary only if the clocked
-            # group reads it (the read must see the value before the edge)
-            nbv = [v for v in sorted(acc.nb_writes, key=lambda x: x.id)
-                   if not v.is_mem and (self.nb_needed is None or v in self.nb_needed)]
-            for v in nbv:
-                self.declare(v)
-                t = f"nb{v.id}_{uid}"
-                if is_wide(v.width):
-                    self.emit(f"uint32_t {t}[{v.words}];")
-                    if v in acc.need_old:
-                        self.emit(f"rfw_copy({t}, {self.name(v)}, {v.width});")
-                else:
-                    if v in acc.need_old:
-                        self.emit(f"{cty(v.width)} {t} = {self.get(v)};")
-                    else:
-                        self.emit(f"{cty(v.width)} {t} = 0;")
-                self.nb[v] = t
-            self.stmts(p.stmts)
-            for v in nbv:
-                t = self.nb.pop(v)
-                if is_wide(v.width):
-                    self.emit(f"rfw_copy({self.name(v)}, {t}, {v.width});")
-                else:
-                    self.emit(self.set(v, t))
-                self.dirty.add(v)
-            self.commit_mem_nb()
+            self.seq_group([p], [acc], uid)
         else:
             # comb: vars whose old value is needed must be loaded first
             for v in sorted(acc.need_old, key=lambda x: x.id):

@@ -147,6 +147,68 @@ def analyze_process(p: Process) -> Access:
     return acc
 
 
+def split_seq_processes(design: Design) -> int:
+    """Split clocked blocks into groups of independent top-level statements
+    (Verilator's V3Split role for always blocks): statements stay together
+    when they write the same variable or memory, or when one reads a variable
+    another one assigns with a blocking assignment. Reads of non-blocking
+    targets across groups become anti-dependencies in the eval DAG, which
+    preserves the read-old-value semantics. Returns the number of new
+    processes."""
+    out: List[Process] = []
+    added = 0
+    for p in design.processes:
+        if p.kind != "seq" or len(p.stmts) < 2:
+            out.append(p)
+            continue
+        info = [analyze_process(Process("seq", [s], p.edges)) for s in p.stmts]
+        n = len(info)
+        parent = list(range(n))
+
+        def find(x):
+            while parent[x] != x:
+                parent[x] = parent[parent[x]]
+                x = parent[x]
+            return x
+
+        def union(a, b):
+            ra, rb = find(a), find(b)
+            if ra != rb:
+                parent[max(ra, rb)] = min(ra, rb)
+
+        owner: Dict[Var, int] = {}
+        for i, acc in enumerate(info):
+            for v in acc.writes:
+                if v in owner:
+                    union(i, owner[v])
+                else:
+                    owner[v] = i
+        bw: Dict[Var, int] = {}
+        for i, acc in enumerate(info):
+            for v in acc.b_writes:
+                if v in bw:
+                    union(i, bw[v])
+                else:
+                    bw[v] = i
+        for i, acc in enumerate(info):
+            for v in acc.dep_reads | acc.need_old:
+                j = bw.get(v)
+                if j is not None and j != i:
+                    union(i, j)
+        groups: Dict[int, List] = {}
+        for i, s in enumerate(p.stmts):
+            groups.setdefault(find(i), []).append(s)
+        if len(groups) == 1:
+            out.append(p)
+            continue
+        for k, (_, stmts) in enumerate(sorted(groups.items())):
+            out.append(Process("seq", stmts, list(p.edges), name=f"{p.name}#{k}", line=p.line))
+        added += len(groups) - 1
+    design.processes = out
+    design.finalize()
+    return added
+
+
 # ----------------------------------------------------------------------------
 # Schedule
 # ----------------------------------------------------------------------------
