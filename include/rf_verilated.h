@@ -114,6 +114,13 @@ int rf_set_mem_images(rf_model* m, uint32_t mem, uint32_t nstim, uint32_t nelem,
  * *bytes receives the size; dst may be NULL to query it. */
 int rf_lane_state(rf_model* m, uint32_t lane, void* dst, size_t* bytes);
 
+/* Time every mtask kernel of one eval in isolation (mean of ``reps``
+ * back-to-back launches over all lanes, HIP events on the model's stream);
+ * ms_out[nmtasks]. The measured costs train the learned cost model; no
+ * counterpart in the reference (its --prof-threads instrumentation is
+ * disabled in the GPU emitter, src/V3EmitC.cpp:1591-1613). */
+int rf_profile_mtasks(rf_model* m, uint32_t reps, float* ms_out);
+
 /* Wait for all queued work; device flags (RF_FLAG_*) of the last run. */
 int rf_sync(rf_model* m);
 int rf_flags(rf_model* m, uint32_t* out, uint32_t n);

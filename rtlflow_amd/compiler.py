@@ -27,7 +27,8 @@ class Options:
     def __init__(self, threads: int = 8, block: int = BLOCK, lane_major_min_depth: int = 64,
                  cost_fn: Optional[Callable] = None, settle_unroll: int = 2,
                  converge_limit: int = 100, state_in_regs: bool = True, keep=None,
-                 mtask_stats: bool = False, inline_limit: int = 6000, split_always: bool = True):
+                 mtask_stats: bool = False, inline_limit: int = 6000, split_always: bool = True,
+                 mtask_waves: int = 4):
         self.threads = threads            # --threads: partition granularity (cpLimit)
         self.block = block
         self.lane_major_min_depth = lane_major_min_depth
@@ -40,6 +41,8 @@ class Options:
         self.inline_limit = inline_limit
         # split clocked blocks into independent statement groups
         self.split_always = split_always
+        # per-mtask kernels: minimum waves per SIMD (bounds their registers)
+        self.mtask_waves = mtask_waves
 
     def key(self) -> str:
         return f"t{self.threads}_b{self.block}_lm{self.lane_major_min_depth}"
@@ -239,12 +242,15 @@ class Compiled:
 # Kernel emission
 # =====================================================================
 
-def kernel_header(comp: Compiled, name: str) -> str:
+def kernel_header(comp: Compiled, name: str, occupancy: int = 0) -> str:
     """Device kernel, or for the host build a per-lane C function with the
-    same body."""
+    same body. ``occupancy``: minimum waves per SIMD the register allocation
+    must leave room for (0 = compiler's choice)."""
     if comp.host:
         return f'extern "C" void {name}(rf_args a, uint32_t lane_in) {{'
-    return f'extern "C" __global__ void __launch_bounds__({comp.opts.block}) {name}(rf_args a) {{'
+    occ = f" __attribute__((amdgpu_waves_per_eu({occupancy})))" if occupancy else ""
+    return (f'extern "C" __global__ void __launch_bounds__({comp.opts.block}){occ} '
+            f'{name}(rf_args a) {{')
 
 
 def lane_expr(comp: Compiled) -> str:
@@ -357,19 +363,11 @@ def _mtask_gen(comp: Compiled, k: int) -> Gen:
 
 
 def emit_mtask_kernel(comp: Compiled, k: int) -> str:
-    if comp.calls:
-        # thin wrapper over the shared mtask body
-        lines = [kernel_header(comp, f"rf_mtask_{k}")]
-        lines.append(f"    const uint32_t lane = {lane_expr(comp)};")
-        lines.append("    if (lane >= a.nlanes) return;")
-        lines.append("    if (a.done[lane]) return;")
-        if comp.sch.change_vars:
-            lines.append("    if (!a.change[lane]) return;")
-        lines.append(f"    rf_body_{k}(a, lane);")
-        lines.append("}")
-        return "\n".join(lines)
+    # the graph-node kernel keeps its own inlined copy of the body (a call to
+    # the shared non-inlined body would cost the call ABI's registers and
+    # scratch in every launch)
     gen = _mtask_gen(comp, k)
-    lines = [kernel_header(comp, f"rf_mtask_{k}")]
+    lines = [kernel_header(comp, f"rf_mtask_{k}", occupancy=comp.opts.mtask_waves)]
     _prologue(comp, gen, lines, check_done=True, check_change=bool(comp.sch.change_vars))
     lines += gen.top_lines
     lines += gen.lines

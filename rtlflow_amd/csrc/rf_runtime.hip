@@ -639,6 +639,29 @@ int rf_lane_state(rf_model* m, uint32_t lane, void* dst, size_t* bytes) {
     return RF_OK;
 }
 
+int rf_profile_mtasks(rf_model* m, uint32_t reps, float* ms_out) {
+    if (!m || !ms_out || !reps) return fail(RF_E_ARG, "bad argument");
+    int rc = do_init(m);
+    if (rc) return rc;
+    const rf_args a = args_for(m, RF_PHASE_EVAL);
+    for (uint32_t k = 0; k < m->d->nmtasks; ++k) {
+        // one warm-up launch, then ``reps`` timed launches back to back
+        rc = launch(m, m->d->mtasks[k].kernel, a);
+        if (rc) return rc;
+        RF_HIP(hipEventRecord(m->ev0, m->stream));
+        for (uint32_t r = 0; r < reps; ++r) {
+            rc = launch(m, m->d->mtasks[k].kernel, a);
+            if (rc) return rc;
+        }
+        RF_HIP(hipEventRecord(m->ev1, m->stream));
+        RF_HIP(hipEventSynchronize(m->ev1));
+        float ms = 0;
+        RF_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+        ms_out[k] = ms / (float)reps;
+    }
+    return RF_OK;
+}
+
 int rf_sync(rf_model* m) {
     if (!m) return fail(RF_E_ARG, "null model");
     RF_HIP(hipStreamSynchronize(m->stream));

@@ -69,6 +69,7 @@ def runtime() -> C.CDLL:
         "rf_submit_queue": (C.c_int, [P, u64, P, u32, P, P]),
         "rf_set_mem_images": (C.c_int, [P, u32, u32, u32, P]),
         "rf_lane_state": (C.c_int, [P, u32, P, C.POINTER(C.c_size_t)]),
+        "rf_profile_mtasks": (C.c_int, [P, u32, P]),
         "rf_sync": (C.c_int, [P]),
         "rf_flags": (C.c_int, [P, P, u32]),
         "rf_get_stats": (C.c_int, [P, C.POINTER(rf_stats)]),
@@ -114,6 +115,10 @@ class DesignLib:
     @property
     def name(self) -> str:
         return self.manifest["name"]
+
+    @property
+    def nmtasks(self) -> int:
+        return int(self.manifest["mtasks"])
 
 
 class Model:
@@ -228,6 +233,12 @@ class Model:
         buf = (C.c_uint8 * n.value)()
         _check(self.rt.rf_lane_state(self.h, lane, buf, C.byref(n)))
         return bytes(buf)
+
+    def profile_mtasks(self, reps: int = 20) -> np.ndarray:
+        """Mean kernel time (ms) of every mtask of one eval, each timed alone."""
+        out = np.zeros(max(1, self.design.nmtasks), dtype=np.float32)
+        _check(self.rt.rf_profile_mtasks(self.h, reps, out.ctypes.data))
+        return out[:self.design.nmtasks]
 
     def flags(self) -> np.ndarray:
         out = np.zeros(8, dtype=np.uint32)

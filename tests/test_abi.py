@@ -24,6 +24,16 @@ def test_runtime_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_runtime_exports_cost_model_symbols():
+    lib = ctypes.CDLL(str(build_runtime()))
+    names = declared("rf_mlp.h")
+    assert "rf_mlp_train_step" in names and "rf_mlp_forward" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    lib.rf_mlp_param_count.restype = ctypes.c_uint32
+    assert lib.rf_mlp_param_count(6) == 64 * 6 + 64 + 64 * 64 + 64 + 64 + 1
+
+
 def test_error_path_without_design():
     lib = ctypes.CDLL(str(build_runtime()))
     lib.rf_create.restype = ctypes.c_int
