@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define RF_DESIGN_ABI 3
+#define RF_DESIGN_ABI 4
 
 /* storage classes (include/rf_heavy.h:11-17: CData/SData/IData/QData) */
 enum { RF_CLS_C = 0, RF_CLS_S = 1, RF_CLS_I = 2, RF_CLS_Q = 3 };
@@ -119,6 +119,10 @@ typedef struct rf_design {
     /* algorithmic bytes per stimulus-cycle (layout manifest) */
     uint64_t bytes_seq, bytes_mem, bytes_in, bytes_out;
     uint64_t ops_per_cycle;    /* estimated instructions per stimulus-cycle */
+    /* bit-plane form: the design lowered to single-bit logic for 64 lanes
+     * per 64-bit word (NULL when the design is outside that subset) */
+    const uint32_t* bitplane;
+    uint64_t bitplane_words;
 } rf_design;
 
 typedef const rf_design* (*rf_design_get_fn)(void);

@@ -34,7 +34,7 @@ enum {
 };
 
 /* run_cycles modes */
-enum { RF_MODE_GRAPH = 0, RF_MODE_FUSED = 1 };
+enum { RF_MODE_GRAPH = 0, RF_MODE_FUSED = 1, RF_MODE_BITPLANE = 2 };
 
 typedef struct rf_config {
     uint32_t settle_unroll;   /* graph form: settle iterations captured per eval (>= 1) */
@@ -50,6 +50,8 @@ typedef struct rf_stats {
     double fused_ms;          /* their summed duration (HIP events on the model's stream) */
     uint64_t graph_launches;  /* graph replays */
     double graph_ms;          /* summed duration of graph replays (HIP events) */
+    uint64_t bitplane_launches; /* bit-plane kernel launches */
+    double bitplane_ms;       /* their summed duration (HIP events) */
 } rf_stats;
 
 /* Library / design ABI versions. */
@@ -91,7 +93,9 @@ int rf_eval(rf_model* m);
  * 0 and its stimulus id to stim0 + lane; rf_run_cycles runs n cycles
  * (inputs for the lane's cycle with clk=0, eval, clk=1, eval, hash outputs).
  * mode RF_MODE_GRAPH replays the captured per-mtask graph once per cycle,
- * RF_MODE_FUSED runs the fused per-lane kernel. */
+ * RF_MODE_FUSED runs the fused per-lane kernel, RF_MODE_BITPLANE runs the
+ * bit-plane kernel (64 lanes per 64-bit word; RF_E_STATE if the design has
+ * no bit-plane program). All three leave identical state and hashes. */
 int rf_begin(rf_model* m, uint64_t seed, uint32_t stim0);
 int rf_run_cycles(rf_model* m, uint32_t n, uint32_t mode);
 int rf_hashes(rf_model* m, uint32_t lane0, uint32_t count, uint64_t* out);

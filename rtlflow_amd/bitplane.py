@@ -1,0 +1,859 @@
+"""Bit-plane form: a design lowered to single-bit logic, evaluated for 64
+stimuli per 64-bit word.
+
+For designs whose logic is bit-level in nature (gate netlists: AND / OR /
+XOR / NAND / MUX over 1-bit signals, constant selects, concatenations), the
+per-lane form spends one 8-bit slot and several instructions per gate per
+stimulus. Here every signal bit becomes a *plane*: one 64-bit word whose bit
+``t`` is the value for stimulus ``t`` of a 64-stimulus group (one wavefront
+of lanes). One gate is then one bit-select instruction pair for 64 stimuli.
+
+``lower(comp)`` symbolically executes the design's scheduled processes into a
+hash-consed DAG of single-bit nodes (AND, XOR, MUX over complementable
+literals), following the cycle protocol the per-lane kernels implement:
+
+  phase A  inputs of cycle c, clk = 0: combinational logic -> next state (D)
+  commit   state <- D for active lanes (the clk = 1 edge: every clocked block
+           reads pre-edge values, non-blocking semantics)
+  phase B  clk = 1: combinational logic -> hashed output bits
+  phase F  after the last cycle: every stored variable, for the write-back
+           to the lane-minor state arrays
+
+Each phase keeps only the cone of what it produces and is levelised; the
+runtime kernel (csrc/rf_bitplane.hip) evaluates one level per barrier with
+every node value in LDS. Nodes take LDS slots by linear-scan allocation over
+levels. Anything outside the supported subset (memories, multiplies wider
+than 16 bits, division, combinational cycles, several clocks, negedge
+blocks, latches) raises ``Unsupported`` and the design keeps only the
+per-lane forms.
+
+``encode(prog)`` packs the program into the u32 blob the design library
+exports (``rf_design.bitplane``); its layout is documented on ``encode``.
+"""
+from __future__ import annotations
+
+from typing import Dict, List, Optional, Sequence, Tuple
+
+from .ir import (Assign, Case, Concat, Cond, Const, Expr, If, Lhs, MemRd, Op, Ref, Sel, Var)
+
+Lit = int          # 2 * node + inverted
+F, T = 0, 1        # constant literals (node 0 is the constant 0)
+
+MAX_SLOTS = 20000  # 160 KB of LDS at 8 bytes per slot
+MAX_MUL_WIDTH = 16
+
+
+class Unsupported(Exception):
+    pass
+
+
+class Net:
+    """Hash-consed single-bit DAG. Node kinds: 'c0', 'pi', 'st', 'and',
+    'xor', 'mux'. Operands are literals."""
+
+    def __init__(self):
+        self.kind: List[str] = ["c0"]
+        self.ops: List[Tuple[int, ...]] = [()]
+        self.info: List[object] = [None]
+        self.memo: Dict[Tuple, int] = {}
+
+    def leaf(self, kind: str, info) -> Lit:
+        n = len(self.kind)
+        self.kind.append(kind)
+        self.ops.append(())
+        self.info.append(info)
+        return 2 * n
+
+    def _node(self, kind: str, ops: Tuple[int, ...]) -> Lit:
+        key = (kind,) + ops
+        n = self.memo.get(key)
+        if n is None:
+            n = len(self.kind)
+            self.kind.append(kind)
+            self.ops.append(ops)
+            self.info.append(None)
+            self.memo[key] = n
+        return 2 * n
+
+    # -------------------------------------------------------------- gates
+    def AND(self, a: Lit, b: Lit) -> Lit:
+        if a == F or b == F or a == (b ^ 1):
+            return F
+        if a == T:
+            return b
+        if b == T or a == b:
+            return a
+        if a > b:
+            a, b = b, a
+        return self._node("and", (a, b))
+
+    def OR(self, a: Lit, b: Lit) -> Lit:
+        return self.AND(a ^ 1, b ^ 1) ^ 1
+
+    def XOR(self, a: Lit, b: Lit) -> Lit:
+        inv = (a & 1) ^ (b & 1)
+        a &= ~1
+        b &= ~1
+        if a == b:
+            return inv
+        if a == F:
+            return b ^ inv
+        if b == F:
+            return a ^ inv
+        if a > b:
+            a, b = b, a
+        return self._node("xor", (a, b)) ^ inv
+
+    def NOT(self, a: Lit) -> Lit:
+        return a ^ 1
+
+    def MUX(self, s: Lit, a: Lit, b: Lit) -> Lit:
+        """s ? a : b"""
+        if s == T:
+            return a
+        if s == F:
+            return b
+        if a == b:
+            return a
+        if s & 1:
+            s, a, b = s ^ 1, b, a
+        if a == T and b == F:
+            return s
+        if a == F and b == T:
+            return s ^ 1
+        if a == F:
+            return self.AND(s ^ 1, b)
+        if b == F:
+            return self.AND(s, a)
+        if a == T:
+            return self.OR(s, b)
+        if b == T:
+            return self.OR(s ^ 1, a)
+        if a == (b ^ 1):
+            return self.XOR(s, b)
+        if a == s:
+            return self.OR(s, b)
+        if a == (s ^ 1):
+            return self.AND(s ^ 1, b)
+        if b == s:
+            return self.AND(s, a)
+        if b == (s ^ 1):
+            return self.OR(s ^ 1, a)
+        inv = a & 1
+        return self._node("mux", (s, a ^ inv, b ^ inv)) ^ inv
+
+    # ------------------------------------------------------ word helpers
+    def const(self, width: int, value: int) -> List[Lit]:
+        return [T if (value >> i) & 1 else F for i in range(width)]
+
+    def redor(self, bits: Sequence[Lit]) -> Lit:
+        return self._tree(list(bits), self.OR, F)
+
+    def redand(self, bits: Sequence[Lit]) -> Lit:
+        return self._tree(list(bits), self.AND, T)
+
+    def redxor(self, bits: Sequence[Lit]) -> Lit:
+        return self._tree(list(bits), self.XOR, F)
+
+    def _tree(self, xs: List[Lit], f, unit: Lit) -> Lit:
+        if not xs:
+            return unit
+        while len(xs) > 1:
+            nxt = [f(xs[i], xs[i + 1]) for i in range(0, len(xs) - 1, 2)]
+            if len(xs) & 1:
+                nxt.append(xs[-1])
+            xs = nxt
+        return xs[0]
+
+    def add(self, a: Sequence[Lit], b: Sequence[Lit], cin: Lit = F) -> List[Lit]:
+        out = []
+        c = cin
+        for x, y in zip(a, b):
+            t = self.XOR(x, y)
+            out.append(self.XOR(t, c))
+            c = self.MUX(t, c, x)
+        return out
+
+    def borrow(self, a: Sequence[Lit], b: Sequence[Lit]) -> Lit:
+        """1 iff a < b (unsigned): the carry out of a + ~b + 1 is 0."""
+        c = T
+        for x, y in zip(a, b):
+            t = self.XOR(x, y ^ 1)
+            c = self.MUX(t, c, x)
+        return c ^ 1
+
+    def shift(self, bits: List[Lit], amt: List[Lit], left: bool, fill: Lit) -> List[Lit]:
+        w = len(bits)
+        x = list(bits)
+        k = 0
+        while k < len(amt) and (1 << k) < w:
+            s = 1 << k
+            if left:
+                y = [fill] * s + x[:w - s]
+            else:
+                y = x[s:] + [fill] * s
+            x = [self.MUX(amt[k], yi, xi) for xi, yi in zip(x, y)]
+            k += 1
+        # any higher amount bit set: everything shifted out
+        hi = amt[k:]
+        if hi:
+            big = self.redor(hi)
+            x = [self.MUX(big, fill, xi) for xi in x]
+        return x
+
+
+def _fit(bits: List[Lit], w: int) -> List[Lit]:
+    return bits[:w] + [F] * (w - len(bits))
+
+
+class Lowerer:
+    def __init__(self, comp):
+        self.comp = comp
+        self.d = comp.design
+        self.net = Net()
+        self.pi: Dict[Var, List[Lit]] = {}
+        self.state: Dict[Var, List[Lit]] = {}
+
+    # ----------------------------------------------------------- exprs
+    def ev(self, e: Expr, env: Dict[Var, List[Lit]]) -> List[Lit]:
+        n = self.net
+        if isinstance(e, Const):
+            return n.const(e.width, e.value)
+        if isinstance(e, Ref):
+            v = env.get(e.var)
+            if v is None:
+                raise Unsupported(f"read of {e.var.name} before any value")
+            return list(v)
+        if isinstance(e, MemRd):
+            raise Unsupported("memory read")
+        if isinstance(e, Sel):
+            a = self.ev(e.a, env)
+            if isinstance(e.lsb, int):
+                return _fit(a[e.lsb:], e.width)
+            amt = self.ev(e.lsb, env)
+            wide = a + [F] * e.width
+            return n.shift(wide, amt, left=False, fill=F)[:e.width]
+        if isinstance(e, Concat):
+            out: List[Lit] = []
+            for p in reversed(e.parts):
+                out += self.ev(p, env)
+            return out
+        if isinstance(e, Cond):
+            c = self.ev(e.c, env)[0]
+            a = self.ev(e.a, env)
+            b = self.ev(e.b, env)
+            return [n.MUX(c, x, y) for x, y in zip(a, b)]
+        if isinstance(e, Op):
+            return self.op(e, env)
+        raise Unsupported(f"expression {type(e).__name__}")
+
+    def op(self, e: Op, env) -> List[Lit]:
+        n = self.net
+        o = e.op
+        args = [self.ev(x, env) for x in e.args]
+        w = e.width
+        if o == "not":
+            return [x ^ 1 for x in args[0]]
+        if o in ("and", "or", "xor", "xnor"):
+            f = {"and": n.AND, "or": n.OR, "xor": n.XOR,
+                 "xnor": lambda x, y: n.XOR(x, y) ^ 1}[o]
+            return [f(x, y) for x, y in zip(args[0], args[1])]
+        if o == "add":
+            return n.add(args[0], args[1])
+        if o == "sub":
+            return n.add(args[0], [x ^ 1 for x in args[1]], T)
+        if o == "neg":
+            return n.add(n.const(w, 0), [x ^ 1 for x in args[0]], T)
+        if o == "mul":
+            if w > MAX_MUL_WIDTH:
+                raise Unsupported("wide multiply")
+            acc = n.const(w, 0)
+            a, b = args
+            for i in range(w):
+                part = [F] * i + [n.AND(x, b[i]) for x in a[:w - i]]
+                acc = n.add(acc, part)
+            return acc
+        if o in ("divu", "modu", "divs", "mods"):
+            raise Unsupported("division")
+        if o == "redand":
+            return [n.redand(args[0])]
+        if o == "redor":
+            return [n.redor(args[0])]
+        if o == "redxor":
+            return [n.redxor(args[0])]
+        if o == "lognot":
+            return [n.redor(args[0]) ^ 1]
+        if o == "logand":
+            return [n.AND(n.redor(args[0]), n.redor(args[1]))]
+        if o == "logor":
+            return [n.OR(n.redor(args[0]), n.redor(args[1]))]
+        if o == "zext":
+            return _fit(args[0], w)
+        if o == "sext":
+            a = args[0]
+            return a + [a[-1]] * (w - len(a))
+        if o in ("eq", "ne"):
+            d = n.redor([n.XOR(x, y) for x, y in zip(args[0], args[1])])
+            return [d ^ 1 if o == "eq" else d]
+        if o in ("ltu", "leu", "gtu", "geu", "lts", "les", "gts", "ges"):
+            a, b = args
+            if o[-1] == "s":
+                a = a[:-1] + [a[-1] ^ 1]
+                b = b[:-1] + [b[-1] ^ 1]
+            base = o[:2]
+            if base == "lt":
+                r = n.borrow(a, b)
+            elif base == "gt":
+                r = n.borrow(b, a)
+            elif base == "le":
+                r = n.borrow(b, a) ^ 1
+            else:
+                r = n.borrow(a, b) ^ 1
+            return [r]
+        if o in ("shl", "shr", "sra"):
+            a, amt = args
+            fill = a[-1] if o == "sra" else F
+            if isinstance(e.args[1], Const):
+                k = e.args[1].value
+                if k >= w:
+                    return [fill] * w
+                if o == "shl":
+                    return [F] * k + a[:w - k]
+                return a[k:] + [fill] * k
+            return n.shift(a, amt, left=(o == "shl"), fill=fill)
+        raise Unsupported(f"operator {o}")
+
+    # ------------------------------------------------------- statements
+    def write(self, env, lhs: Lhs, val: List[Lit], rd_env) -> None:
+        v = lhs.var
+        if v.is_mem or lhs.idx is not None:
+            raise Unsupported("memory write")
+        if lhs.whole:
+            env[v] = list(val)
+            return
+        old = env.get(v)
+        if old is None:
+            raise Unsupported(f"partial write of {v.name} without a prior value")
+        old = list(old)
+        if isinstance(lhs.lsb, int):
+            for i, b in enumerate(val):
+                if lhs.lsb + i < v.width:
+                    old[lhs.lsb + i] = b
+            env[v] = old
+            return
+        n = self.net
+        amt = self.ev(lhs.lsb, rd_env)
+        shifted = n.shift(_fit(val, v.width), amt, left=True, fill=F)
+        m = n.shift(_fit([T] * lhs.width, v.width), amt, left=True, fill=F)
+        env[v] = [n.MUX(mi, si, oi) for mi, si, oi in zip(m, shifted, old)]
+
+    def merge(self, c: Lit, a: Dict[Var, List[Lit]], b: Dict[Var, List[Lit]],
+              base: Dict[Var, List[Lit]], keep_old: bool = False) -> Dict[Var, List[Lit]]:
+        """Join two branches. A variable set on one branch only keeps its old
+        value on the other: the pre-edge state for non-blocking writes
+        (``keep_old``), otherwise a latch, which this form does not model."""
+        n = self.net
+        out = dict(base)
+        for v in set(a) | set(b):
+            x, y = a.get(v), b.get(v)
+            if x is y:
+                out[v] = x
+                continue
+            if keep_old:
+                x = x if x is not None else self.state_bits(v)
+                y = y if y is not None else self.state_bits(v)
+            elif x is None or y is None:
+                raise Unsupported(f"{v.name} assigned on one branch only (latch)")
+            out[v] = [n.MUX(c, p, q) for p, q in zip(x, y)]
+        return out
+
+    def run(self, stmts, env, nba, seq: bool):
+        """Execute statements symbolically. env: blocking view (reads and
+        blocking writes); nba: pending non-blocking values (seq only)."""
+        n = self.net
+        for s in stmts:
+            if isinstance(s, Assign):
+                val = self.ev(s.rhs, env)
+                if s.nb:
+                    if not seq:
+                        raise Unsupported("non-blocking assignment in combinational logic")
+                    if s.lhs.var not in nba:
+                        nba[s.lhs.var] = list(self.state_bits(s.lhs.var))
+                    self.write(nba, s.lhs, val, env)
+                else:
+                    self.write(env, s.lhs, val, env)
+            elif isinstance(s, If):
+                c = self.ev(s.cond, env)[0]
+                self._branch(c, s.then, s.els, env, nba, seq)
+            elif isinstance(s, Case):
+                x = self.ev(s.expr, env)
+                self._case(x, s.items, 0, env, nba, seq)
+            else:
+                raise Unsupported(f"statement {type(s).__name__}")
+
+    def _branch(self, c, then, els, env, nba, seq):
+        e1, n1 = dict(env), dict(nba)
+        self.run(then, e1, n1, seq)
+        e2, n2 = dict(env), dict(nba)
+        self.run(els, e2, n2, seq)
+        env.update(self.merge(c, e1, e2, env))
+        nba.update(self.merge(c, n1, n2, nba, keep_old=True))
+
+    def _case(self, x, items, k, env, nba, seq):
+        n = self.net
+        if k >= len(items):
+            return
+        labels, body = items[k]
+        if labels is None:
+            self.run(body, env, nba, seq)
+            return
+        hits = []
+        for value, care in labels:
+            d = [n.XOR(b, T if (value >> i) & 1 else F) for i, b in enumerate(x) if (care >> i) & 1]
+            hits.append(n.redor(d) ^ 1)
+        c = n.redor(hits)
+        e1, n1 = dict(env), dict(nba)
+        self.run(body, e1, n1, seq)
+        e2, n2 = dict(env), dict(nba)
+        self._case(x, items, k + 1, e2, n2, seq)
+        env.update(self.merge(c, e1, e2, env))
+        nba.update(self.merge(c, n1, n2, nba, keep_old=True))
+
+    def state_bits(self, v: Var) -> List[Lit]:
+        if v not in self.state:
+            raise Unsupported(f"non-blocking write to {v.name}, which is not clocked state")
+        return self.state[v]
+
+    # -------------------------------------------------------- the design
+    def comb_env(self, clk_val: Lit) -> Dict[Var, List[Lit]]:
+        comp = self.comp
+        env: Dict[Var, List[Lit]] = {}
+        # variables no process drives and nothing stores read as zero
+        for v in self.d.vars:
+            if not v.is_mem:
+                env[v] = [F] * v.width
+        env.update(self.pi)
+        env.update(self.state)
+        for c in self.d.clocks:
+            env[c] = [clk_val]
+        for vid in comp.sch.topo:
+            vx = comp.sch.vertices[vid]
+            if vx.kind != "comb":
+                continue
+            for p in vx.procs:
+                local = dict(env)
+                for v in vx.writes:
+                    local.pop(v, None)
+                self.run(p.stmts, local, {}, seq=False)
+                for v in p.writes:
+                    if v.is_mem:
+                        raise Unsupported("memory")
+                    if v not in local:
+                        raise Unsupported(f"{v.name} not assigned")
+                    env[v] = local[v]
+        return env
+
+    def lower(self):
+        comp = self.comp
+        d = self.d
+        sch = comp.sch
+        n = self.net
+        if sch.change_vars:
+            raise Unsupported("combinational cycle (settle loop)")
+        if len(d.clocks) > 1:
+            raise Unsupported("several clocks")
+        if any(v.is_mem for v in comp.stored):
+            raise Unsupported("memory")
+        clk = d.clocks[0] if d.clocks else None
+        seq_vx = [sch.vertices[vid] for vid in sch.topo if sch.vertices[vid].kind == "seq"]
+        for vx in seq_vx:
+            for p in vx.procs:
+                if any(edge != "posedge" or c is not clk for edge, c in p.edges):
+                    raise Unsupported("clocked block not on the posedge of the clock")
+        # leaves
+        for v in d.vars:
+            if v.kind == "input" and v not in d.clocks:
+                self.pi[v] = [n.leaf("pi", (v, i)) for i in range(v.width)]
+        # state: every stored variable that no combinational block drives
+        # (clocked registers, and registers only initial blocks set),
+        # loaded from the lane-minor arrays when a launch starts
+        comb_written = set()
+        for vx in sch.vertices:
+            if vx.kind == "comb":
+                comb_written |= vx.writes
+        synth = set(comp.clklast.values())
+        state_vars = [v for v in d.vars if v in comp.stored and not v.is_mem and v.kind != "input"
+                      and v not in comb_written and v not in synth]
+        for v in state_vars:
+            self.state[v] = [n.leaf("st", (v, i)) for i in range(v.width)]
+        # phase A: clk = 0 settle, then the clocked blocks read pre-edge values
+        envA = self.comb_env(F)
+        nxt: Dict[Var, List[Lit]] = {}
+        written_by: Dict[Var, int] = {}
+        for vx in seq_vx:
+            for p in vx.procs:
+                for v in p.writes:
+                    if v in written_by and written_by[v] != p.id:
+                        raise Unsupported(f"{v.name} written by several clocked blocks")
+                    written_by[v] = p.id
+                env = dict(envA)
+                for c in d.clocks:
+                    env[c] = [T]
+                nba: Dict[Var, List[Lit]] = {}
+                self.run(p.stmts, env, nba, seq=True)
+                for v in p.writes:
+                    if v in nba:
+                        nxt[v] = nba[v]
+                    elif v in self.state:
+                        # blocking write of a reg used only inside its block
+                        readers = [q for q in d.processes if v in q.reads and q is not p]
+                        if readers:
+                            raise Unsupported(f"blocking write of shared {v.name} in a clocked block")
+                        nxt[v] = env[v]
+        for v in state_vars:
+            if v not in nxt:
+                nxt[v] = self.state[v]
+        # phase B / F: clk = 1 settle on the committed state
+        envB = self.comb_env(T)
+        self.envB = envB
+        self.nxt = nxt
+        self.state_vars = state_vars
+        return self
+
+
+# =====================================================================
+# Levelisation and slot allocation
+# =====================================================================
+
+class Phase:
+    def __init__(self):
+        self.levels: List[List[Tuple[int, int, int, int, int, int]]] = []
+
+
+class Program:
+    pass
+
+
+def _cone(net: Net, roots: Sequence[Lit]) -> List[int]:
+    """Internal nodes reachable from roots, in topological order."""
+    seen = set()
+    order = []
+    stack = [(r >> 1, False) for r in roots]
+    while stack:
+        x, done = stack.pop()
+        if done:
+            order.append(x)
+            continue
+        if x in seen:
+            continue
+        seen.add(x)
+        if net.kind[x] in ("c0", "pi", "st"):
+            continue
+        stack.append((x, True))
+        for o in net.ops[x]:
+            if (o >> 1) not in seen:
+                stack.append((o >> 1, False))
+    return order
+
+
+def build_phase(net: Net, roots: Sequence[Lit], fixed: Dict[int, int], nfixed: int,
+                keep: Sequence[Lit] = ()) -> Tuple[Phase, Dict[int, int], int]:
+    """Levelise the cone of ``roots`` and give each node an LDS slot.
+    ``fixed``: node -> slot for leaves. Slots of ``keep`` nodes stay live to
+    the end of the phase. Returns (phase, node->slot, peak slots)."""
+    order = _cone(net, roots)
+    level: Dict[int, int] = {}
+    for x in order:
+        level[x] = 1 + max((level.get(o >> 1, 0) for o in net.ops[x]), default=0)
+    nlev = max(level.values(), default=0)
+    last: Dict[int, int] = {}
+    for x in order:
+        for o in net.ops[x]:
+            y = o >> 1
+            if y in level:
+                last[y] = max(last.get(y, 0), level[x])
+    END = nlev + 1
+    for r in list(roots) + list(keep):
+        if (r >> 1) in level:
+            last[r >> 1] = END
+    by_level: List[List[int]] = [[] for _ in range(nlev + 1)]
+    for x in order:
+        by_level[level[x]].append(x)
+    free: List[int] = []
+    top = nfixed
+    slot: Dict[int, int] = dict(fixed)
+    release: Dict[int, List[int]] = {}
+    for x, l in last.items():
+        release.setdefault(l, []).append(x)
+    peak = nfixed
+    ph = Phase()
+    for L in range(1, nlev + 1):
+        gates = []
+        for x in by_level[L]:
+            if free:
+                s = free.pop()
+            else:
+                s = top
+                top += 1
+            slot[x] = s
+            gates.append(_gate(net, x, slot))
+        peak = max(peak, top)
+        # sort by operation shape so neighbouring lanes take the same path
+        gates.sort(key=lambda g: (g[1], g[2]))
+        ph.levels.append(gates)
+        for y in release.get(L, []):
+            if y in slot and y not in fixed:
+                free.append(slot[y])
+    return ph, slot, peak
+
+
+def _lit_slot(lit: Lit, slot: Dict[int, int]) -> Tuple[int, int]:
+    return slot[lit >> 1], lit & 1
+
+
+def _gate(net: Net, x: int, slot: Dict[int, int]) -> Tuple[int, int, int, int, int, int]:
+    """(dst, s, xs, ix, ys, iy): dst = s ? (xs ^ ix) : (ys ^ iy)."""
+    k = net.kind[x]
+    ops = net.ops[x]
+    if k == "and":
+        a, b = ops
+        s, inv = _lit_slot(a, slot)
+        bs, bi = _lit_slot(b, slot)
+        if inv:      # ~a ? b : 0  ==  a ? 0 : b
+            return (slot[x], s, 0, 0, bs, bi)
+        return (slot[x], s, bs, bi, 0, 0)
+    if k == "xor":
+        a, b = ops
+        s, _ = _lit_slot(a, slot)
+        bs, bi = _lit_slot(b, slot)
+        return (slot[x], s, bs, bi ^ 1, bs, bi)
+    if k == "copy":
+        src, inv = _lit_slot(ops[0], slot)
+        return (slot[x], src, 0, inv ^ 1, 0, inv)
+    if k == "mux":
+        sl, a, b = ops
+        s, _ = _lit_slot(sl, slot)
+        xs, xi = _lit_slot(a, slot)
+        ys, yi = _lit_slot(b, slot)
+        return (slot[x], s, xs, xi, ys, yi)
+    raise AssertionError(k)
+
+
+CLS_ID = {"C": 0, "S": 1, "I": 2, "Q": 3, "W": 2}
+CLS_SIZE = {"C": 1, "S": 2, "I": 4, "Q": 8, "W": 4}
+
+
+def compile_program(comp) -> Program:
+    """Lower and schedule; raises Unsupported for designs outside the subset."""
+    from .layout import class_of
+    lw = Lowerer(comp).lower()
+    net = lw.net
+    d = comp.design
+    lay = comp.layout
+    # fixed slots: 0 = zeros, then input bits, then state bits
+    fixed: Dict[int, int] = {0: 0}
+    top = 1
+    in_chunks = []
+    for port in comp.port_table():
+        if port["dir"] != 0:
+            continue
+        v = port["var"]
+        bits = lw.pi[v]
+        for j in range((v.width + 63) // 64):
+            nb = min(64, v.width - 64 * j)
+            in_chunks.append(((port["stim_index"] << 3) | j, nb, top))
+            for i in range(nb):
+                fixed[bits[64 * j + i] >> 1] = top + i
+            top += nb
+    state_load = []
+    for v in lw.state_vars:
+        bits = lw.state[v]
+        state_load.append((CLS_ID[class_of(v)], lay.loc[v], v.slots, v.width, top))
+        for i, b in enumerate(bits):
+            fixed[b >> 1] = top + i
+        top += v.width
+    nfixed = top
+    # phase A: next-state bits. A next-state literal that is a leaf or
+    # inverted gets a copy node so commit reads a slot of its own.
+    commit_lits = []
+    for v in lw.state_vars:
+        for i, lit in enumerate(lw.nxt[v]):
+            commit_lits.append((lw.state[v][i] >> 1, lit))
+    copies = {}
+    roots = []
+    for st_node, lit in commit_lits:
+        node = lit >> 1
+        if (lit & 1) or net.kind[node] in ("c0", "pi", "st"):
+            # a node of its own holding the literal's value (not hash-consed)
+            key = lit
+            if key not in copies:
+                n = len(net.kind)
+                net.kind.append("copy")
+                net.ops.append((lit,))
+                net.info.append(None)
+                copies[key] = 2 * n
+            roots.append(copies[key])
+        else:
+            roots.append(lit)
+    phA, slotA, peakA = build_phase(net, roots, fixed, nfixed)
+    commit = [(fixed[st_node], slotA[r >> 1]) for (st_node, _), r in zip(commit_lits, roots)]
+    # phase B: hashed outputs after the edge
+    out_ports = [p for p in comp.port_table() if p["hashed"]]
+    hroots = [b for p in out_ports for b in lw.envB[p["var"]]]
+    phB, slotB, peakB = build_phase(net, hroots, fixed, nfixed)
+    hash_ports = []
+    for p in out_ports:
+        v = p["var"]
+        nbytes = CLS_SIZE[class_of(v)] * (v.slots if v.width > 64 else 1)
+        hash_ports.append((nbytes, v.width, [_lit_ref(b, slotB) for b in lw.envB[v]]))
+    # phase F: every stored variable after the last cycle
+    wb = []
+    froots = []
+    for v in d.vars:
+        if v.is_mem or not lay.has(v):
+            continue
+        bits = _final_bits(comp, lw, v)
+        if bits is None:
+            continue
+        wb.append((v, bits))
+        froots += bits
+    phF, slotF, peakF = build_phase(net, froots, fixed, nfixed)
+    writeback = [(CLS_ID[class_of(v)], lay.loc[v], v.slots, v.width, [_lit_ref(b, slotF) for b in bits])
+                 for v, bits in wb]
+    prog = Program()
+    prog.nslots = max(peakA, peakB, peakF)
+    if prog.nslots > MAX_SLOTS:
+        raise Unsupported(f"{prog.nslots} live bit slots exceed LDS")
+    prog.in_chunks = in_chunks
+    prog.state_load = state_load
+    prog.phases = [phA, phB, phF]
+    prog.commit = commit
+    prog.hash_ports = hash_ports
+    prog.writeback = writeback
+    prog.nodes = len(net.kind)
+    prog.gates = [sum(len(l) for l in ph.levels) for ph in prog.phases]
+    prog.nlevels = [len(ph.levels) for ph in prog.phases]
+    return prog
+
+
+def _lit_ref(lit: Lit, slot: Dict[int, int]) -> int:
+    """Slot of a literal with the inversion in bit 31."""
+    return slot[lit >> 1] | ((lit & 1) << 31)
+
+
+def _final_bits(comp, lw: Lowerer, v: Var) -> Optional[List[Lit]]:
+    d = comp.design
+    if v in d.clocks or v in comp.clklast.values():
+        return [T]
+    if v in lw.state:
+        return lw.state[v]
+    if v in lw.envB:
+        return lw.envB[v]
+    return None
+
+
+# =====================================================================
+# Encoding
+# =====================================================================
+
+BP_MAGIC = 0x50424652   # "RFBP"
+BP_VERSION = 1
+HDR_WORDS = 16
+
+
+def gate_word(g: Tuple[int, int, int, int, int, int]) -> int:
+    dst, s, xs, ix, ys, iy = g
+    return dst | (s << 15) | (xs << 30) | (ix << 45) | (ys << 46) | (iy << 61)
+
+
+def encode(prog: Program) -> List[int]:
+    """u32 words of the program blob.
+
+    header (16 words): magic, version, nslots, n_in, n_state, n_commit,
+    n_hash, n_wb, nlev[A, B, F], ngates[A, B, F], total words, 0.
+    Then, each section starting at an even word offset:
+      inputs      n_in  x {k, nbits, slot0}: bits of stimulus stream k into
+                  slots slot0 .. slot0 + nbits - 1
+      state load  n_state x {cls, loc_lo, loc_hi, words, width, slot0}
+      commit      n_commit x {state slot, next-state slot}
+      phase A/B/F level offsets (nlev + 1 gate indices, padded to even),
+                  then the gates as u64 (lo word first):
+                  dst | s << 15 | x << 30 | ix << 45 | y << 46 | iy << 61,
+                  dst = s ? (x ^ ix) : (y ^ iy) on 64-lane planes
+      hash        n_hash x {nbytes, width, width slot refs}
+      write-back  n_wb x {cls, loc_lo, loc_hi, words, width, width slot refs}
+    A slot ref is a slot index with the inversion in bit 31.
+    """
+    out: List[int] = [0] * HDR_WORDS
+
+    def pad():
+        if len(out) & 1:
+            out.append(0)
+
+    pad()
+    for k, nb, s0 in prog.in_chunks:
+        out += [k, nb, s0]
+    pad()
+    for cls, loc, words, width, s0 in prog.state_load:
+        out += [cls, loc & 0xffffffff, loc >> 32, words, width, s0]
+    pad()
+    for st, dn in prog.commit:
+        out += [st, dn]
+    for ph in prog.phases:
+        pad()
+        offs = [0]
+        for lev in ph.levels:
+            offs.append(offs[-1] + len(lev))
+        out += offs
+        pad()
+        for lev in ph.levels:
+            for g in lev:
+                w = gate_word(g)
+                out += [w & 0xffffffff, w >> 32]
+    pad()
+    for nbytes, width, refs in prog.hash_ports:
+        out += [nbytes, width] + refs
+    pad()
+    for cls, loc, words, width, refs in prog.writeback:
+        out += [cls, loc & 0xffffffff, loc >> 32, words, width] + refs
+    pad()
+    hdr = [BP_MAGIC, BP_VERSION, prog.nslots, len(prog.in_chunks), len(prog.state_load),
+           len(prog.commit), len(prog.hash_ports), len(prog.writeback)]
+    hdr += [len(ph.levels) for ph in prog.phases]
+    hdr += [sum(len(l) for l in ph.levels) for ph in prog.phases]
+    hdr += [len(out), 0]
+    out[:HDR_WORDS] = hdr
+    return out
+
+
+def sections(blob: Sequence[int]) -> Dict[str, int]:
+    """Word offsets of every section of an encoded program (the runtime
+    computes the same offsets from the header)."""
+    h = blob[:HDR_WORDS]
+    assert h[0] == BP_MAGIC and h[1] == BP_VERSION
+    n_in, n_state, n_commit, n_hash, n_wb = h[3:8]
+    nlev = h[8:11]
+    ngates = h[11:14]
+    o = HDR_WORDS
+    ev = lambda x: x + (x & 1)
+    res = {}
+    res["in"] = o = ev(o)
+    o += 3 * n_in
+    res["state"] = o = ev(o)
+    o += 6 * n_state
+    res["commit"] = o = ev(o)
+    o += 2 * n_commit
+    for k, name in enumerate("ABF"):
+        res["lev" + name] = o = ev(o)
+        o += nlev[k] + 1
+        res["gates" + name] = o = ev(o)
+        o += 2 * ngates[k]
+    res["hash"] = o = ev(o)
+    for _ in range(n_hash):
+        o += 2 + blob[o + 1]
+    res["wb"] = o = ev(o)
+    for _ in range(n_wb):
+        o += 5 + blob[o + 4]
+    res["end"] = ev(o)
+    assert res["end"] == h[14], (res["end"], h[14])
+    return res

@@ -57,8 +57,9 @@ def runtime_path() -> Path:
 
 def build_runtime(force: bool = False) -> Path:
     out = runtime_path()
-    srcs = [CSRC / "rf_runtime.hip", CSRC / "rf_mlp.hip"]
-    deps = srcs + [INCLUDE / "rf_verilated.h", INCLUDE / "rf_design.h", INCLUDE / "rf_mlp.h"]
+    srcs = [CSRC / "rf_runtime.hip", CSRC / "rf_mlp.hip", CSRC / "rf_bitplane.hip"]
+    deps = srcs + [INCLUDE / "rf_verilated.h", INCLUDE / "rf_design.h", INCLUDE / "rf_mlp.h",
+                   CSRC / "rf_bitplane.h", CSRC / "rf_stim.h"]
     if out.exists() and not force and all(out.stat().st_mtime >= p.stat().st_mtime for p in deps):
         return out
     BUILD.mkdir(parents=True, exist_ok=True)
@@ -113,6 +114,26 @@ def build_design(name: str, sources: Dict[str, str], manifest: dict, jobs: int =
     for o in objs:
         o.unlink()
     return lib
+
+
+def build_testbench(lib: Path, manifest: dict) -> Path:
+    """Write ``V<top>.h`` (the generated C++ model class) and the host-driven
+    testbench ``V<top>_tb.cpp`` beside a built design library, and link the
+    testbench against it and the runtime. Returns the executable."""
+    from .emit_vtop import emit_testbench, emit_vtop_header
+    d = lib.parent
+    nm = manifest["name"]
+    (d / f"V{nm}.h").write_text(emit_vtop_header(manifest))
+    tb = d / f"V{nm}_tb.cpp"
+    tb.write_text(emit_testbench(manifest))
+    exe = d / f"V{nm}_tb"
+    rt = build_runtime()
+    rel = os.path.relpath(rt.parent, d)
+    _run(["g++", "-O2", "-std=c++17", "-I", str(INCLUDE), "-I", str(CSRC), "-I", str(d), str(tb),
+          "-o", str(exe), "-L", str(d), f"-lV{nm}", "-L", str(rt.parent), "-lrf_verilated",
+          "-Wl,-rpath-link,/opt/rocm/lib",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/" + rel], log=d / "tb.log")
+    return exe
 
 
 def emit_makefiles(outdir: Path, top: str, sources: List[str]):

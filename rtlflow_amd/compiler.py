@@ -28,7 +28,7 @@ class Options:
                  cost_fn: Optional[Callable] = None, settle_unroll: int = 2,
                  converge_limit: int = 100, state_in_regs: bool = True, keep=None,
                  mtask_stats: bool = False, inline_limit: int = 6000, split_always: bool = True,
-                 mtask_waves: int = 4):
+                 mtask_waves: int = 4, bitplane: bool = True):
         self.threads = threads            # --threads: partition granularity (cpLimit)
         self.block = block
         self.lane_major_min_depth = lane_major_min_depth
@@ -43,6 +43,8 @@ class Options:
         self.split_always = split_always
         # per-mtask kernels: minimum waves per SIMD (bounds their registers)
         self.mtask_waves = mtask_waves
+        # also lower the design to the bit-plane form when it is in that subset
+        self.bitplane = bitplane
 
     def key(self) -> str:
         return f"t{self.threads}_b{self.block}_lm{self.lane_major_min_depth}"
@@ -93,6 +95,24 @@ class Compiled:
         # inline form (all logic in one function, state in registers) for
         # small designs; shared non-inlined mtask bodies for large ones
         self.calls = sum(self.vcost) > self.opts.inline_limit
+        self._bitplane()
+
+    def _bitplane(self):
+        """Bit-plane program (rtlflow_amd.bitplane) or None with the reason."""
+        self.bitplane_blob = None
+        self.bitplane_info: dict = {"supported": False, "reason": "disabled"}
+        if not self.opts.bitplane:
+            return
+        from . import bitplane as bp
+        try:
+            prog = bp.compile_program(self)
+        except bp.Unsupported as e:
+            self.bitplane_info = {"supported": False, "reason": str(e)}
+            return
+        self.bitplane_blob = bp.encode(prog)
+        self.bitplane_info = {"supported": True, "slots": prog.nslots, "gates": prog.gates,
+                              "levels": prog.nlevels, "words": len(self.bitplane_blob),
+                              "state_bits": sum(x[3] for x in prog.state_load)}
 
     # ---------------------------------------------------------- storage
     def _storage(self):
@@ -190,6 +210,7 @@ class Compiled:
             "threads": self.opts.threads,
             "block": self.opts.block,
             "form": "calls" if self.calls else "inline",
+            "bitplane": self.bitplane_info,
         }
 
     def mem_table(self) -> List[dict]:
@@ -966,8 +987,6 @@ def emit_descriptor(comp: Compiled, h: int) -> str:
     if not mems:
         out.append("    {0}")
     out.append("};")
-    for k, mt in enumerate(comp.mtasks):
-        preds = sorted(comp.mtasks.index(p) if False else p for p in [])
     # mtask predecessor lists (indices into the topologically ordered array)
     id2idx = {mt.id: k for k, mt in enumerate(comp.mtasks)}
     for k, mt in enumerate(comp.mtasks):
@@ -987,6 +1006,13 @@ def emit_descriptor(comp: Compiled, h: int) -> str:
             clk_idx = i
             break
     lay = comp.layout
+    if comp.bitplane_blob:
+        # bit-plane program (layout: rtlflow_amd/bitplane.py, encode)
+        out.append("static const uint32_t rf_bitplane[] = {")
+        blob = comp.bitplane_blob
+        for i in range(0, len(blob), 16):
+            out.append("    " + ", ".join(f"{x}u" for x in blob[i:i + 16]) + ",")
+        out.append("};")
     out.append("static const rf_design rf_the_design = {")
     out.append(f"    RF_DESIGN_ABI, {comp.opts.block}u, \"{d.name}\", 0x{h:x}ull,")
     out.append(f"    {{{lay.slots['C']}ull, {lay.slots['S']}ull, {lay.slots['I']}ull, {lay.slots['Q']}ull}},")
@@ -996,10 +1022,18 @@ def emit_descriptor(comp: Compiled, h: int) -> str:
     out.append("    (const void*)&rf_k_fused, (const void*)&rf_k_persistent, (const void*)&rf_k_reset,")
     out.append(f"    {len(comp.sch.change_vars)}u, {clk_idx}u,")
     out.append(f"    {man['bytes_seq']}ull, 0ull, {man['bytes_in']}ull, {man['bytes_out']}ull,")
-    out.append(f"    {man['ops_per_cycle']}ull")
+    out.append(f"    {man['ops_per_cycle']}ull,")
+    if comp.bitplane_blob:
+        out.append(f"    rf_bitplane, {len(comp.bitplane_blob)}ull")
+    else:
+        out.append("    0, 0ull")
     out.append("};")
     # the only exported symbol: kernels stay library-local so several design
     # libraries can live in one process
     out.append('extern "C" __attribute__((visibility("default"))) const rf_design* '
                'rf_design_get(void) { return &rf_the_design; }')
+    # the same table under a design-specific name, for the generated C++ model
+    # class (several designs may then be linked into one program)
+    out.append(f'extern "C" __attribute__((visibility("default"))) const rf_design* '
+               f'rf_design_get_{d.name}(void) {{ return &rf_the_design; }}')
     return "\n".join(out)

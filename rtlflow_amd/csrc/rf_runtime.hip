@@ -23,6 +23,7 @@
 #include <string>
 #include <vector>
 
+#include "rf_bitplane.h"
 #include "rf_verilated.h"
 
 namespace {
@@ -79,6 +80,8 @@ struct rf_model {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     rf_stats stats{};
     rf_args base{};
+    uint32_t* bp_prog = nullptr;   // device copy of the bit-plane program
+    rf_bp_params bp{};
 };
 
 namespace {
@@ -173,6 +176,34 @@ int build_graphs(rf_model* m) {
     rc = add_eval(m, m->cycle_graph, t1, unroll, RF_PHASE_HASH, argstore, &t2);
     if (rc) return rc;
     RF_HIP(hipGraphInstantiate(&m->cycle_exec, m->cycle_graph, nullptr, nullptr, 0));
+    return RF_OK;
+}
+
+// Check the design's bit-plane program against the layout and copy it to
+// the device.
+int load_bitplane(rf_model* m) {
+    const rf_design* d = m->d;
+    const uint32_t* b = d->bitplane;
+    const char* why = nullptr;
+    if (rf_bp_parse(b, d->bitplane_words, &m->bp, &why)) return fail(RF_E_DESIGN, "%s", why);
+    const rf_bp_params& p = m->bp;
+    for (uint32_t k = 0; k < p.n_state; ++k) {
+        const uint32_t* sl = b + p.off_state + 6 * k;
+        const uint64_t loc = (uint64_t)sl[1] | ((uint64_t)sl[2] << 32);
+        const uint64_t n = sl[4] > 64 ? sl[3] : 1;
+        if (loc + n > d->slots[sl[0]]) return fail(RF_E_DESIGN, "bit-plane state slot outside the layout");
+    }
+    uint64_t q = p.off_wb;
+    for (uint32_t k = 0; k < p.n_wb; ++k) {
+        const uint64_t loc = (uint64_t)b[q + 1] | ((uint64_t)b[q + 2] << 32);
+        const uint64_t n = b[q + 4] > 64 ? b[q + 3] : 1;
+        if (loc + n > d->slots[b[q]]) return fail(RF_E_DESIGN, "bit-plane write-back slot outside the layout");
+        q += 5 + b[q + 4];
+    }
+    const size_t bytes = d->bitplane_words * 4;
+    RF_HIP(hipMalloc(&m->bp_prog, bytes));
+    RF_HIP(hipMemcpy(m->bp_prog, b, bytes, hipMemcpyHostToDevice));
+    m->bp.prog = m->bp_prog;
     return RF_OK;
 }
 
@@ -296,6 +327,10 @@ int rf_create(const rf_design* d, uint32_t n_lanes, const rf_config* cfg, rf_mod
     if (e != hipSuccess) return cleanup(fail(RF_E_HIP, "hipEventCreate: %s", hipGetErrorString(e)));
     int rc = build_graphs(m);
     if (rc) return cleanup(rc);
+    if (d->bitplane && d->bitplane_words) {
+        rc = load_bitplane(m);
+        if (rc) return cleanup(rc);
+    }
     *out = m;
     return RF_OK;
 }
@@ -310,7 +345,7 @@ int rf_destroy(rf_model* m) {
     for (int c = 0; c < 4; ++c)
         if (m->cls[c]) (void)hipFree(m->cls[c]);
     void* ps[] = {m->mem, m->change, m->done, m->hash, m->cyc, m->stim, m->flags,
-                  m->budget, m->order, m->out_hash, m->images};
+                  m->budget, m->order, m->out_hash, m->images, m->bp_prog};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     for (void* p : m->image_bufs)
@@ -510,6 +545,24 @@ int rf_run_cycles(rf_model* m, uint32_t n, uint32_t mode) {
             RF_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
             m->stats.fused_launches++;
             m->stats.fused_ms += ms;
+            left -= chunk;
+        }
+    } else if (mode == RF_MODE_BITPLANE) {
+        if (!m->bp_prog) return fail(RF_E_STATE, "design %s has no bit-plane program", m->d->name);
+        uint32_t left = n;
+        while (left) {
+            const uint32_t chunk = std::min(left, m->cfg.fused_chunk);
+            rf_bp_params p = m->bp;
+            p.a = args_for(m, RF_PHASE_EVAL);
+            p.a.ncycles = chunk;
+            RF_HIP(hipEventRecord(m->ev0, m->stream));
+            RF_HIP(rf_bp_launch(p, m->stream));
+            RF_HIP(hipEventRecord(m->ev1, m->stream));
+            RF_HIP(hipEventSynchronize(m->ev1));
+            float ms = 0;
+            RF_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+            m->stats.bitplane_launches++;
+            m->stats.bitplane_ms += ms;
             left -= chunk;
         }
     } else {

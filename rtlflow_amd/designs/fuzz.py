@@ -17,7 +17,8 @@ WIDTHS = [1, 3, 7, 8, 13, 16, 31, 32, 33, 47, 63, 64, 65, 96, 128, 130]
 
 
 class Gen:
-    def __init__(self, seed: int, wide: bool = True):
+    def __init__(self, seed: int, wide: bool = True, bitlevel: bool = False):
+        self.bitlevel = bitlevel
         self.r = random.Random(seed)
         self.wide = wide
         self.sigs: List[Tuple[str, int, bool]] = []   # name, width, signed
@@ -47,6 +48,8 @@ class Gen:
         if k < 0.4 and w > 4:
             pw = r.randint(1, min(w, 16))
             st, _ = self.narrow_expr(2)
+            if self.bitlevel:
+                return f"{name}[{st} +: {pw}]", pw
             return f"{name}[({st}) % {w} +: {pw}]", pw
         if s:
             return f"$signed({name})", w
@@ -69,12 +72,14 @@ class Gen:
             b, wb = self.expr(depth - 1)
             if op == "*" and max(wa, wb) > 64 and r.random() < 0.5:
                 op = "+"
+            if op == "*" and self.bitlevel:
+                op = "&"
             return f"({a} {op} {b})", max(wa, wb)
         if k < 0.42:
             op = r.choice(["/", "%"])
             a, wa = self.expr(depth - 1)
             b, wb = self.expr(depth - 1)
-            if max(wa, wb) > 64:
+            if max(wa, wb) > 64 or self.bitlevel:
                 return f"({a} ^ {b})", max(wa, wb)
             return f"({a} {op} {b})", max(wa, wb)
         if k < 0.52:
@@ -119,8 +124,10 @@ class Gen:
 
 
 def verilog(seed: int, n_in: int = 5, n_wire: int = 8, n_reg: int = 6, depth: int = 3,
-            wide: bool = True, name: str = "fuzz") -> str:
-    g = Gen(seed, wide)
+            wide: bool = True, name: str = "fuzz", bitlevel: bool = False) -> str:
+    """``bitlevel``: no memory, no division, multiplies at most 16 bits wide
+    (the subset the bit-plane form lowers)."""
+    g = Gen(seed, wide, bitlevel)
     r = g.r
     L = []
     ports = ["input clk"]
@@ -138,15 +145,17 @@ def verilog(seed: int, n_in: int = 5, n_wire: int = 8, n_reg: int = 6, depth: in
         regs.append((f"r{i}", w, s))
     g.sigs.extend(regs)
     mw = g.width() if wide else r.choice([8, 16, 32, 64])
-    body.append(f"    reg [{mw - 1}:0] mem [0:15];")
+    if not bitlevel:
+        body.append(f"    reg [{mw - 1}:0] mem [0:15];")
     for i in range(n_wire):
         e, w = g.expr(depth)
         wdecl = r.choice([w, g.width()])
         s = r.random() < 0.2
         body.append(f"    wire {'signed ' if s else ''}[{wdecl - 1}:0] w{i} = {e};")
         g.sigs.append((f"w{i}", wdecl, s))
-    body.append(f"    wire [{mw - 1}:0] mrd = mem[w0[3:0]];")
-    g.sigs.append(("mrd", mw, False))
+    if not bitlevel:
+        body.append(f"    wire [{mw - 1}:0] mrd = mem[w0[3:0]];")
+        g.sigs.append(("mrd", mw, False))
     body.append("    always @(posedge clk) begin")
     for name_, w, _ in regs:
         e, _ = g.expr(depth)
@@ -169,7 +178,8 @@ def verilog(seed: int, n_in: int = 5, n_wire: int = 8, n_reg: int = 6, depth: in
         else:
             body.append(f"        {name_} <= {e};")
     e, _ = g.expr(2)
-    body.append(f"        if (w1[0]) mem[i0[3:0]] <= {e};")
+    if not bitlevel:
+        body.append(f"        if (w1[0]) mem[i0[3:0]] <= {e};")
     body.append("    end")
     picks = [s for s in g.sigs if not s[0].startswith("i")]
     outs = r.sample(picks, min(6, len(picks)))

@@ -91,6 +91,9 @@ def build_entry(name: str, jobs: int = 8) -> dict:
     cd = compile_design(spec["verilog"], top=spec["top"], name=name, jobs=jobs, **e.opts)
     rec = {"name": name, "lib": str(cd.lib_path.relative_to(BUILD)), "opts": e.opts,
            "build_s": round(time.time() - t0, 1)}
+    from ..build import build_testbench
+    exe = build_testbench(cd.lib_path, cd.manifest)
+    rec["testbench"] = str(exe.relative_to(BUILD))
     if "table" in spec:
         tp = cd.lib_path.parent / "gates.bin"
         tp.write_bytes(spec["table"])
@@ -158,3 +161,16 @@ def _fuzz(seed: int):
 # random operator-coverage designs (compared with the IR interpreter)
 for _s in (1, 2, 3, 4):
     register(f"fuzz{_s}", _fuzz(_s), threads=3)
+
+
+def _bfuzz(seed: int):
+    def make():
+        from .fuzz import verilog
+        return {"verilog": verilog(seed, name=f"bfuzz{seed}", bitlevel=True, depth=4),
+                "top": f"bfuzz{seed}"}
+    return make
+
+
+# random designs inside the bit-plane subset (no memories, no division)
+for _s in (1, 2, 3):
+    register(f"bfuzz{_s}", _bfuzz(_s), threads=2)

@@ -21,6 +21,7 @@ from .build import runtime_path
 
 MODE_GRAPH = 0
 MODE_FUSED = 1
+MODE_BITPLANE = 2
 
 _DT = {0: np.uint8, 1: np.uint16, 2: np.uint32, 3: np.uint64}
 
@@ -36,7 +37,8 @@ class rf_config(C.Structure):
 
 class rf_stats(C.Structure):
     _fields_ = [("fused_launches", C.c_uint64), ("fused_ms", C.c_double),
-                ("graph_launches", C.c_uint64), ("graph_ms", C.c_double)]
+                ("graph_launches", C.c_uint64), ("graph_ms", C.c_double),
+                ("bitplane_launches", C.c_uint64), ("bitplane_ms", C.c_double)]
 
 
 _rt = None
@@ -249,7 +251,8 @@ class Model:
         s = rf_stats()
         _check(self.rt.rf_get_stats(self.h, C.byref(s)))
         return {"fused_launches": s.fused_launches, "fused_ms": s.fused_ms,
-                "graph_launches": s.graph_launches, "graph_ms": s.graph_ms}
+                "graph_launches": s.graph_launches, "graph_ms": s.graph_ms,
+                "bitplane_launches": s.bitplane_launches, "bitplane_ms": s.bitplane_ms}
 
     def reset_stats(self):
         _check(self.rt.rf_reset_stats(self.h))
