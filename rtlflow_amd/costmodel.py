@@ -254,3 +254,149 @@ class LearnedCost:
             self._cache[key] = {v.id: max(1, int(round(u * self.scale))) for v, u in
                                 zip(comp.sch.vertices, us)}
         return self._cache[key][vx.id]
+
+
+# =====================================================================
+# Training data, training, prediction
+# =====================================================================
+#
+# 1. build_variants() (CPU): partition variants of registered designs
+#    (threads 2..16 give mtasks of every size); each library gets the
+#    features of its mtasks, and the targets the features of their vertices.
+# 2. profile_variants() (GPU): every mtask kernel timed alone at a fixed lane
+#    count (rf_profile_mtasks).
+# 3. fit() (GPU): the MLP learns log(us) from log1p(features).
+# 4. vertex_costs(): predicted marginal cost of every vertex of a target,
+#    written as a cost table that TableCost feeds to the partitioner.
+
+TRAIN_DESIGNS = ("net_small", "net2", "alu", "systolic", "rv32i", "fuzz1", "fuzz2", "fuzz3", "fuzz4",
+                 "bfuzz1", "bfuzz2", "bfuzz3")
+TRAIN_THREADS = (2, 4, 8, 16)
+TARGETS = ("net2", "systolic", "rv32i")
+PROFILE_LANES = 16384
+
+
+def _cm_dir():
+    from .build import BUILD
+    return BUILD / "costmodel"
+
+
+def build_variants(names=TRAIN_DESIGNS, threads=TRAIN_THREADS, jobs: int = 8, log=print) -> dict:
+    import json
+    from . import compile_design
+    from .build import BUILD
+    from .designs import registry
+    index = {}
+    for name in names:
+        e = registry.ENTRIES[name]
+        spec = e.make()
+        for t in threads:
+            opts = dict(e.opts)
+            opts["threads"] = t
+            opts["bitplane"] = False
+            vname = f"{name}_t{t}"
+            cd = compile_design(spec["verilog"], top=spec["top"], name=vname, jobs=jobs, **opts)
+            comp = cd.compiled
+            fx = FeatureExtractor(comp)
+            rec = {"design": name, "threads": t, "lib": str(cd.lib_path.relative_to(BUILD)),
+                   "mtask_features": [fx.mtask(k) for k in range(len(comp.mtasks))]}
+            if name in TARGETS and t == e.opts.get("threads"):
+                rec["vertex_features"] = [fx.vertex(v.id) for v in comp.sch.vertices]
+            (cd.lib_path.parent / "features.json").write_text(json.dumps(rec))
+            index[vname] = rec["lib"]
+            if log:
+                log(f"variant {vname}: {len(comp.mtasks)} mtasks")
+    d = _cm_dir()
+    d.mkdir(parents=True, exist_ok=True)
+    (d / "index.json").write_text(json.dumps(index, indent=1))
+    return index
+
+
+def profile_variants(lanes: int = PROFILE_LANES, reps: int = 20, log=print) -> List[dict]:
+    """GPU: time every mtask of every variant (mean of ``reps`` launches)."""
+    import json
+    from .build import BUILD
+    from .model import MODE_GRAPH, DesignLib, Model
+    index = json.loads((_cm_dir() / "index.json").read_text())
+    samples = []
+    for vname, lib in index.items():
+        path = BUILD / lib
+        rec = json.loads((path.parent / "features.json").read_text())
+        dl = DesignLib(path)
+        n = lanes
+        with Model(dl, n) as m:
+            m.begin(0x5EED)
+            m.run_cycles(4, MODE_GRAPH)
+            ms = m.profile_mtasks(reps)
+        for k, f in enumerate(rec["mtask_features"]):
+            samples.append({"variant": vname, "design": rec["design"], "threads": rec["threads"],
+                            "mtask": k, "features": f, "us": float(ms[k]) * 1e3})
+        if log:
+            log(f"profiled {vname}: {len(rec['mtask_features'])} mtasks, "
+                f"{float(ms.sum()) * 1e3:.1f} us per eval")
+    return samples
+
+
+def fit(samples: List[dict], epochs: int = 600, seed: int = 1) -> Dict:
+    x = encode([s["features"] for s in samples])
+    y = np.log(np.maximum([s["us"] for s in samples], 1e-3)).astype(np.float32)
+    model = train(x, y, epochs=epochs, lr=2e-3, batch=128, seed=seed)
+    model["features"] = FEATURE_NAMES
+    model["lanes"] = PROFILE_LANES
+    return model
+
+
+def vertex_costs(model: Dict, vertex_features: List[List[float]], scale: float = 100.0) -> List[int]:
+    """Predicted marginal cost of each vertex: predicted time of a kernel
+    holding only that vertex, minus that of an empty kernel (the launch),
+    in units of 1/scale us."""
+    lc = LearnedCost(model)
+    us = lc.predict_us(vertex_features)
+    base = float(lc.predict_us([[0.0] * N_FEATURES])[0])
+    lc.net.close()
+    return [max(1, int(round((u - base) * scale))) for u in us]
+
+
+class TableCost:
+    """Partitioner cost function from a precomputed per-vertex cost table
+    (vertex ids follow the deterministic schedule of the same source)."""
+
+    def __init__(self, costs: List[int]):
+        self.costs = list(costs)
+
+    def __call__(self, comp, vx) -> int:
+        if len(self.costs) != len(comp.sch.vertices):
+            raise ValueError(f"cost table has {len(self.costs)} entries, design has "
+                             f"{len(comp.sch.vertices)} vertices")
+        return self.costs[vx.id]
+
+
+def main(argv=None):
+    """GPU workflow: profile the variants, train, predict the targets' vertex
+    costs. ``python -m rtlflow_amd.costmodel OUTDIR``."""
+    import json
+    import sys
+    from pathlib import Path
+    from .build import BUILD
+    argv = argv if argv is not None else sys.argv[1:]
+    out = Path(argv[0] if argv else "costmodel_out")
+    out.mkdir(parents=True, exist_ok=True)
+    samples = profile_variants(log=lambda *a: print(*a, flush=True))
+    (out / "samples.json").write_text(json.dumps(samples))
+    designs = sorted({s["design"] for s in samples})
+    model = fit(samples)
+    # hold-out check: per design, a model trained without it
+    print(f"trained on {len(samples)} mtasks from {len(designs)} designs: "
+          f"val mse {model['history'][-1][2]:.4f}", flush=True)
+    (out / "model.json").write_text(json.dumps(model))
+    index = json.loads((_cm_dir() / "index.json").read_text())
+    for vname, lib in index.items():
+        rec = json.loads(((BUILD / lib).parent / "features.json").read_text())
+        if "vertex_features" in rec:
+            costs = vertex_costs(model, rec["vertex_features"])
+            (out / f"costs_{rec['design']}.json").write_text(json.dumps(costs))
+            print(f"cost table {rec['design']}: {len(costs)} vertices", flush=True)
+
+
+if __name__ == "__main__":
+    main()

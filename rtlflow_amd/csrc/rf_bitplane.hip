@@ -21,24 +21,41 @@ __device__ __forceinline__ uint64_t bp_ref(const uint64_t* L, uint32_t r) {
     return (r >> 31) ? ~v : v;
 }
 
+// 64 x 64 bit-matrix transpose across wavefront 0: thread j holds row j on
+// entry and column j on exit (bit t of the result = bit j of thread t's
+// row). Six block-swap stages (32, 16, ..., 1), each one cross-lane exchange.
+__device__ __forceinline__ uint64_t bp_shfl_xor64(uint64_t v, uint32_t m) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, (int)m, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), (int)m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t bp_transpose64(uint64_t x, uint32_t t) {
+    const uint64_t lo_masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                                  0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const uint32_t s = 32u >> k;
+        const uint64_t p = bp_shfl_xor64(x, s);
+        const uint64_t lo = lo_masks[k];
+        x = (t & s) ? ((x & ~lo) | ((p & ~lo) >> s)) : ((x & lo) | ((p & lo) << s));
+    }
+    return x;
+}
+
 // Bits [0, nbits) of every lane's v (wavefront 0, all 64 threads) become
 // planes L[s0 .. s0 + nbits).
 __device__ __forceinline__ void bp_to_planes(uint64_t* L, uint32_t s0, uint64_t v, uint32_t nbits,
                                              uint32_t t) {
-    uint64_t mine = 0;
-    for (uint32_t b = 0; b < nbits; ++b) {
-        const uint64_t pl = __ballot((int)((v >> b) & 1ull));
-        mine = (t == b) ? pl : mine;
-    }
-    if (t < nbits) L[s0 + t] = mine;
+    const uint64_t col = bp_transpose64(v, t);
+    if (t < nbits) L[s0 + t] = col;
 }
 
 // This lane's bits of planes refs[0 .. nbits) (nbits <= 64).
 __device__ __forceinline__ uint64_t bp_gather(const uint64_t* L, const uint32_t* refs, uint32_t nbits,
                                               uint32_t t) {
-    uint64_t v = 0;
-    for (uint32_t b = 0; b < nbits; ++b) v |= ((bp_ref(L, refs[b]) >> t) & 1ull) << b;
-    return v;
+    const uint64_t row = t < nbits ? bp_ref(L, refs[t]) : 0ull;
+    return bp_transpose64(row, t);
 }
 
 __device__ __forceinline__ void bp_phase(uint64_t* L, const uint32_t* lev, const uint2* gates,

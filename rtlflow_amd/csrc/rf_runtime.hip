@@ -696,6 +696,12 @@ int rf_profile_mtasks(rf_model* m, uint32_t reps, float* ms_out) {
     if (!m || !ms_out || !reps) return fail(RF_E_ARG, "bad argument");
     int rc = do_init(m);
     if (rc) return rc;
+    // every lane requests a pass, so kernels that skip settled lanes do work
+    {
+        std::vector<uint32_t> ones(m->npad, 1u);
+        RF_HIP(hipMemcpyAsync(m->change, ones.data(), m->npad * 4, hipMemcpyHostToDevice, m->stream));
+        RF_HIP(hipStreamSynchronize(m->stream));
+    }
     const rf_args a = args_for(m, RF_PHASE_EVAL);
     for (uint32_t k = 0; k < m->d->nmtasks; ++k) {
         // one warm-up launch, then ``reps`` timed launches back to back
