@@ -42,14 +42,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(cd, lanes, mode, steps, cps, warmup, seed, chunk):
+def run_form(cd, lanes, mode, steps, cps, warmup, seed, chunk, bp_threads=0):
     if warmup:
-        with cd.model(lanes, fused_chunk=chunk) as m:
+        with cd.model(lanes, fused_chunk=chunk, bitplane_threads=bp_threads) as m:
             m.begin(seed)
             for _ in range(warmup):
                 m.run_cycles(cps, mode)
             m.sync()
-    with cd.model(lanes, fused_chunk=chunk) as m:
+    with cd.model(lanes, fused_chunk=chunk, bitplane_threads=bp_threads) as m:
         m.begin(seed)
         m.sync()
         m.reset_stats()
@@ -73,6 +73,10 @@ def main():
     ap.add_argument("--design", default="net2")
     ap.add_argument("--forms", default="bitplane,fused,graph")
     ap.add_argument("--chunk", type=int, default=512, help="cycles per fused / bit-plane launch")
+    ap.add_argument("--bp-threads", type=int, default=0,
+                    help="bit-plane form: threads per 64-lane workgroup (0: runtime default)")
+    ap.add_argument("--workload", default="netlist", choices=["netlist", "alu", "systolic", "rv32i"],
+                    help="netlist (default, the headline) or one of the other synthetic designs")
     ap.add_argument("--check", type=int, default=64, help="stimuli checked against the oracle")
     ap.add_argument("--cpu-sample", type=int, default=64, help="stimuli in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -99,6 +103,8 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.workload != "netlist":
+        return bench_other(args)
     from rtlflow_amd.designs import registry
     from rtlflow_amd.model import MODE_BITPLANE, MODE_FUSED, MODE_GRAPH
     import oracle
@@ -117,7 +123,8 @@ def main():
             continue
         mode = {"fused": MODE_FUSED, "graph": MODE_GRAPH, "bitplane": MODE_BITPLANE}[form]
         log(f"[bench] {args.design} form={form} lanes={lanes} cycles={cycles}")
-        dt, st, h = run_form(cd, lanes, mode, steps, cps, args.warmup, seed, args.chunk)
+        dt, st, h = run_form(cd, lanes, mode, steps, cps, args.warmup, seed, args.chunk,
+                             args.bp_threads)
         nchk = min(args.check, lanes)
         ref = oracle.netlist_run(registry.table_bytes(args.design), seed, nstim=nchk, cycles=cycles,
                                  threads=16)
@@ -192,6 +199,96 @@ def main():
                      "alg_bytes_per_stimulus_cycle": b_alg, "avg_launch_ms": per_launch_ms},
         "valu": {"scalar_equivalent_ops": valu, "peak_ops": VALU_PEAK, "frac": valu / VALU_PEAK,
                  "ops_per_stimulus_cycle": ops},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_other(args):
+    """The other synthetic designs at their own sizes: ALU 1,024 x 10,000,
+    systolic 8,192 x 5,000 (graph and fused forms), RV32I 16,384 stimuli with
+    lognormal budgets through the batched queue, beside the graph form at the
+    median budget."""
+    from rtlflow_amd.designs import registry, rv32i
+    from rtlflow_amd.model import MODE_FUSED, MODE_GRAPH
+    import oracle
+    wl = args.workload
+    cfg = {"alu": 1, "rv32i": 3, "systolic": 4}[wl]
+    seed = registry.SEED_CFG[cfg]
+    cd = registry.get(wl)
+    man = cd.manifest
+    forms = {}
+    if wl in ("alu", "systolic"):
+        lanes, cycles = {"alu": (1024, 10000), "systolic": (8192, 5000)}[wl]
+        cps = min(cycles, args.cycles_per_step * 10)
+        steps = cycles // cps
+        run = oracle.alu_run if wl == "alu" else oracle.systolic_run
+        nchk = min(args.check, lanes)
+        ref = run(seed, nchk, cycles, threads=16)
+        for form, mode in (("fused", MODE_FUSED), ("graph", MODE_GRAPH)):
+            log(f"[bench] {wl} form={form} lanes={lanes} cycles={cycles}")
+            dt, st, h = run_form(cd, lanes, mode, steps, cps, args.warmup, seed, args.chunk)
+            ok = bool((h[:nchk] == ref).all())
+            forms[form] = {"seconds": dt, "value": lanes * cycles / dt, "bit_exact_sample": ok,
+                           "checked_stimuli": nchk}
+            log(f"[bench] form={form} {dt:.3f}s {lanes * cycles / dt:.4g} stim-cycles/s oracle_match={ok}")
+        workload = f"{wl}: {lanes} stimuli x {cycles} cycles"
+        t0 = time.perf_counter()
+        run(seed, args.cpu_sample, cycles, threads=16)
+        cdt = time.perf_counter() - t0
+        cpu = {"value": args.cpu_sample * cycles / cdt, "unit": "stimulus-cycles/s", "cores": 16,
+               "kind": "port", "sample": f"{args.cpu_sample} stimuli x {cycles} cycles, C oracle"}
+    else:
+        nstim, lanes = 16384, args.lanes if args.lanes != 65536 else 8192
+        progs = rv32i.programs(seed, np.arange(nstim))
+        budgets = rv32i.budgets(seed, nstim)
+        total = int(budgets.astype(np.int64).sum())
+        nchk = min(args.check, nstim)
+        ref = oracle.rv32i_run(progs[:nchk], seed, budgets[:nchk], threads=16)
+        log(f"[bench] rv32i batched queue: {nstim} stimuli on {lanes} lanes, {total} stimulus-cycles")
+        with cd.model(lanes) as m:
+            m.set_mem_images("imem", progs)
+            m.sync()
+            t0 = time.perf_counter()
+            out = m.submit_queue(seed, budgets)
+            dt = time.perf_counter() - t0
+        ok = bool((out[:nchk] == ref).all())
+        forms["batched"] = {"seconds": dt, "value": total / dt, "bit_exact_sample": ok,
+                            "checked_stimuli": nchk, "lanes": lanes}
+        log(f"[bench] batched {dt:.3f}s {total / dt:.4g} stim-cycles/s oracle_match={ok}")
+        med = int(np.median(budgets))
+        gl = 4096
+        with cd.model(gl) as m:
+            m.mem_write("imem", progs[:gl])
+            m.begin(seed)
+            m.sync()
+            t0 = time.perf_counter()
+            m.run_cycles(med, MODE_GRAPH)
+            m.sync()
+            dtg = time.perf_counter() - t0
+            hg = m.hashes()
+        refg = oracle.rv32i_run(progs[:8], seed, [med] * 8, threads=8)
+        forms["graph"] = {"seconds": dtg, "value": gl * med / dtg, "bit_exact_sample":
+                          bool((hg[:8] == refg).all()), "checked_stimuli": 8, "lanes": gl,
+                          "cycles": med}
+        log(f"[bench] graph {dtg:.3f}s {gl * med / dtg:.4g} stim-cycles/s")
+        workload = f"rv32i: {nstim} stimuli, lognormal budgets (median 5000, [1000, 100000]), {total} cycles"
+        ns = args.cpu_sample
+        t0 = time.perf_counter()
+        oracle.rv32i_run(progs[:ns], seed, budgets[:ns], threads=16)
+        cdt = time.perf_counter() - t0
+        cpu = {"value": int(budgets[:ns].astype(np.int64).sum()) / cdt, "unit": "stimulus-cycles/s",
+               "cores": 16, "kind": "port", "sample": f"{ns} stimuli at their budgets, C oracle"}
+    exact = {k: v for k, v in forms.items() if v["bit_exact_sample"]}
+    main_form = max(exact or forms, key=lambda k: forms[k]["value"])
+    f = forms[main_form]
+    line = {
+        "metric": "stimulus-cycles/sec, bit-exact vs Verilator CPU; batched+learned vs plain GPU",
+        "value": f["value"], "unit": "stimulus-cycles/s", "n_gpus": 1, "steps": 1,
+        "warmup": args.warmup, "ms_per_step": f["seconds"] * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "integer", "data": "synthetic",
+        "config": {"workload": workload, "design": wl, "form": main_form, "mtasks": man["mtasks"]},
+        "forms": forms, "bit_exact_sample": all(v["bit_exact_sample"] for v in forms.values()),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -39,8 +39,9 @@ enum { RF_MODE_GRAPH = 0, RF_MODE_FUSED = 1, RF_MODE_BITPLANE = 2 };
 typedef struct rf_config {
     uint32_t settle_unroll;   /* graph form: settle iterations captured per eval (>= 1) */
     uint32_t converge_limit;  /* settle iterations before RF_E_CONVERGE (reference: 100) */
-    uint32_t fused_chunk;     /* fused form: cycles per kernel launch */
-    uint32_t reserved;
+    uint32_t fused_chunk;     /* fused / bit-plane forms: cycles per kernel launch */
+    uint32_t bitplane_threads; /* bit-plane form: threads per 64-lane workgroup
+                                  (128, 256, 512, 1024; 0 = 512) */
 } rf_config;
 
 typedef struct rf_model rf_model;

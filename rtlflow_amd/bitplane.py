@@ -679,33 +679,39 @@ def compile_program(comp) -> Program:
     for v in lw.state_vars:
         for i, lit in enumerate(lw.nxt[v]):
             commit_lits.append((lw.state[v][i] >> 1, lit))
-    copies = {}
+    copies: Dict[Lit, Lit] = {}
+
+    def own(lit: Lit) -> Lit:
+        """A node of its own holding the literal's value (not hash-consed)."""
+        if lit not in copies:
+            n = len(net.kind)
+            net.kind.append("copy")
+            net.ops.append((lit,))
+            net.info.append(None)
+            copies[lit] = 2 * n
+        return copies[lit]
+
     roots = []
     for st_node, lit in commit_lits:
         node = lit >> 1
-        if (lit & 1) or net.kind[node] in ("c0", "pi", "st"):
-            # a node of its own holding the literal's value (not hash-consed)
-            key = lit
-            if key not in copies:
-                n = len(net.kind)
-                net.kind.append("copy")
-                net.ops.append((lit,))
-                net.info.append(None)
-                copies[key] = 2 * n
-            roots.append(copies[key])
-        else:
-            roots.append(lit)
+        roots.append(own(lit) if (lit & 1) or net.kind[node] in ("c0", "pi", "st") else lit)
     phA, slotA, peakA = build_phase(net, roots, fixed, nfixed)
     commit = [(fixed[st_node], slotA[r >> 1]) for (st_node, _), r in zip(commit_lits, roots)]
-    # phase B: hashed outputs after the edge
+    # phase B: hashed outputs after the edge. The hash of cycle n runs while
+    # the inputs of cycle n + 1 are written, so an output bit that is an
+    # input bit is read from a copy.
     out_ports = [p for p in comp.port_table() if p["hashed"]]
-    hroots = [b for p in out_ports for b in lw.envB[p["var"]]]
+    hbits = {}
+    for p in out_ports:
+        v = p["var"]
+        hbits[v] = [own(b & ~1) ^ (b & 1) if net.kind[b >> 1] == "pi" else b for b in lw.envB[v]]
+    hroots = [b for p in out_ports for b in hbits[p["var"]]]
     phB, slotB, peakB = build_phase(net, hroots, fixed, nfixed)
     hash_ports = []
     for p in out_ports:
         v = p["var"]
         nbytes = CLS_SIZE[class_of(v)] * (v.slots if v.width > 64 else 1)
-        hash_ports.append((nbytes, v.width, [_lit_ref(b, slotB) for b in lw.envB[v]]))
+        hash_ports.append((nbytes, v.width, [_lit_ref(b, slotB) for b in hbits[v]]))
     # phase F: every stored variable after the last cycle
     wb = []
     froots = []
@@ -720,6 +726,9 @@ def compile_program(comp) -> Program:
     phF, slotF, peakF = build_phase(net, froots, fixed, nfixed)
     writeback = [(CLS_ID[class_of(v)], lay.loc[v], v.slots, v.width, [_lit_ref(b, slotF) for b in bits])
                  for v, bits in wb]
+    # single-bit entries first: fixed-size records the kernel indexes directly
+    writeback.sort(key=lambda e: e[3] != 1)
+    state_load.sort(key=lambda e: e[3] != 1)
     prog = Program()
     prog.nslots = max(peakA, peakB, peakF)
     if prog.nslots > MAX_SLOTS:
@@ -757,8 +766,8 @@ def _final_bits(comp, lw: Lowerer, v: Var) -> Optional[List[Lit]]:
 # =====================================================================
 
 BP_MAGIC = 0x50424652   # "RFBP"
-BP_VERSION = 1
-HDR_WORDS = 16
+BP_VERSION = 2
+HDR_WORDS = 18
 
 
 def gate_word(g: Tuple[int, int, int, int, int, int]) -> int:
@@ -769,8 +778,10 @@ def gate_word(g: Tuple[int, int, int, int, int, int]) -> int:
 def encode(prog: Program) -> List[int]:
     """u32 words of the program blob.
 
-    header (16 words): magic, version, nslots, n_in, n_state, n_commit,
-    n_hash, n_wb, nlev[A, B, F], ngates[A, B, F], total words, 0.
+    header (18 words): magic, version, nslots, n_in, n_state, n_commit,
+    n_hash, n_wb, nlev[A, B, F], ngates[A, B, F], total words, n_state1,
+    n_wb1, 0 -- the first n_state1 state entries and the first n_wb1
+    write-back entries are single-bit (a write-back record is then 6 words).
     Then, each section starting at an even word offset:
       inputs      n_in  x {k, nbits, slot0}: bits of stimulus stream k into
                   slots slot0 .. slot0 + nbits - 1
@@ -821,7 +832,8 @@ def encode(prog: Program) -> List[int]:
            len(prog.commit), len(prog.hash_ports), len(prog.writeback)]
     hdr += [len(ph.levels) for ph in prog.phases]
     hdr += [sum(len(l) for l in ph.levels) for ph in prog.phases]
-    hdr += [len(out), 0]
+    hdr += [len(out), sum(1 for e in prog.state_load if e[3] == 1),
+            sum(1 for e in prog.writeback if e[3] == 1), 0]
     out[:HDR_WORDS] = hdr
     return out
 

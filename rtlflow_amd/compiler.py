@@ -751,7 +751,7 @@ def emit_init(comp: Compiled) -> str:
     loc |= set(comp.design.clocks) | set(comp.clklast.values()) | set(comp.chglast.values()) | \
         set(sch.change_vars)
     gen.local = loc
-    plan_packing(comp, gen, comp.persistent)
+    plan_packing(comp, gen, comp.stored)
     _initial_body(comp, gen)
     for v in sorted(gen.dirty, key=lambda x: x.id):
         if v in comp.stored:
@@ -786,8 +786,9 @@ def plan_packing(comp: Compiled, gen: Gen, carried: Set[Var]):
     live, far more than the 512 registers a lane has as whole words).
 
     ``carried`` variables keep their value across evals (state, clock latches,
-    ports, combinational values read by clocked logic): each gets a slot of
-    its own. Every other 1-bit variable lives only inside one eval; its slot
+    ports, combinational values read by clocked logic) or are written back to
+    the lane-minor arrays when the kernel ends (every stored variable, so the
+    arrays hold the settled state): each gets a slot of its own. Every other 1-bit variable lives only inside one eval; its slot
     is taken at its defining vertex and freed after its last reader in the
     eval's topological order (linear-scan allocation)."""
     sch = comp.sch
@@ -852,7 +853,7 @@ def emit_fused(comp: Compiled) -> str:
     gen = Gen(comp)
     gen.top_lines = []
     gen.local = _fused_locals(comp)
-    plan_packing(comp, gen, comp.persistent)
+    plan_packing(comp, gen, comp.stored)
     # registers that carry values across evals are loaded once
     for v in sorted(gen.local, key=lambda x: x.id):
         if v in comp.persistent:

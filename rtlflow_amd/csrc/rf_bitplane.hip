@@ -2,10 +2,10 @@
 // logic: the clock-cycle protocol (inputs, clk = 0 settle, clock edge,
 // clk = 1 settle, output hash) for 64 stimuli per 64-bit word.
 //
-// Workgroup = RF_BP_THREADS threads for one group of 64 lanes. Wavefront 0
-// does the per-lane work (stimulus generation, ballots into planes, output
-// hashing, state load and write-back); every thread evaluates gates, one
-// level of the program per barrier, with all values in LDS:
+// One workgroup (128 to 1,024 threads) per group of 64 lanes. Wavefronts 0
+// and 1 do the per-lane work (stimulus generation, transposes into planes,
+// output hashing, state load and write-back); every thread evaluates gates,
+// one level of the program per barrier, with all values in LDS:
 //
 //   dst = s ? (x ^ ix) : (y ^ iy)        (a bit-select per 32-bit half)
 //
@@ -23,7 +23,8 @@ __device__ __forceinline__ uint64_t bp_ref(const uint64_t* L, uint32_t r) {
 
 // 64 x 64 bit-matrix transpose across wavefront 0: thread j holds row j on
 // entry and column j on exit (bit t of the result = bit j of thread t's
-// row). Six block-swap stages (32, 16, ..., 1), each one cross-lane exchange.
+// row). Six block-swap stages (32, 16, ..., 1), each one cross-lane exchange;
+// all 64 threads of the wavefront must execute it together.
 __device__ __forceinline__ uint64_t bp_shfl_xor64(uint64_t v, uint32_t m) {
     const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, (int)m, 64);
     const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), (int)m, 64);
@@ -58,22 +59,41 @@ __device__ __forceinline__ uint64_t bp_gather(const uint64_t* L, const uint32_t*
     return bp_transpose64(row, t);
 }
 
+__device__ __forceinline__ void bp_gate(uint64_t* L, uint2 w) {
+    const uint32_t dst = w.x & 0x7fffu;
+    const uint32_t s = (w.x >> 15) & 0x7fffu;
+    const uint32_t xs = ((w.x >> 30) | (w.y << 2)) & 0x7fffu;
+    const uint32_t ys = (w.y >> 14) & 0x7fffu;
+    const uint64_t ix = 0ull - (uint64_t)((w.y >> 13) & 1u);
+    const uint64_t iy = 0ull - (uint64_t)((w.y >> 29) & 1u);
+    const uint64_t S = L[s];
+    const uint64_t X = L[xs] ^ ix;
+    const uint64_t Y = L[ys] ^ iy;
+    L[dst] = (S & X) | (~S & Y);
+}
+
+// One phase: level l's gates are spread over the workgroup, a barrier after
+// each level. Each thread's first descriptor of the next level is loaded
+// before the current level is evaluated, so its global-memory latency
+// overlaps the current level and the barrier.
+template <int NT>
 __device__ __forceinline__ void bp_phase(uint64_t* L, const uint32_t* lev, const uint2* gates,
                                          uint32_t nlev, uint32_t tid) {
+    if (nlev == 0) return;
+    uint32_t lo = lev[0], hi = lev[1];
+    uint2 nxt = (lo + tid < hi) ? gates[lo + tid] : make_uint2(0u, 0u);
     for (uint32_t l = 0; l < nlev; ++l) {
-        const uint32_t g1 = lev[l + 1];
-        for (uint32_t g = lev[l] + tid; g < g1; g += RF_BP_THREADS) {
-            const uint2 w = gates[g];
-            const uint32_t dst = w.x & 0x7fffu;
-            const uint32_t s = (w.x >> 15) & 0x7fffu;
-            const uint32_t xs = ((w.x >> 30) | (w.y << 2)) & 0x7fffu;
-            const uint32_t ys = (w.y >> 14) & 0x7fffu;
-            const uint64_t ix = 0ull - (uint64_t)((w.y >> 13) & 1u);
-            const uint64_t iy = 0ull - (uint64_t)((w.y >> 29) & 1u);
-            const uint64_t S = L[s];
-            const uint64_t X = L[xs] ^ ix;
-            const uint64_t Y = L[ys] ^ iy;
-            L[dst] = (S & X) | (~S & Y);
+        const uint2 cur = nxt;
+        const uint32_t g0 = lo + tid;
+        const uint32_t g1 = hi;
+        if (l + 1 < nlev) {
+            lo = hi;
+            hi = lev[l + 2];
+            if (lo + tid < hi) nxt = gates[lo + tid];
+        }
+        if (g0 < g1) {
+            bp_gate(L, cur);
+            for (uint32_t g = g0 + NT; g < g1; g += NT) bp_gate(L, gates[g]);
         }
         __syncthreads();
     }
@@ -100,33 +120,96 @@ __device__ __forceinline__ void bp_store(const rf_args& a, uint32_t cls, uint64_
     }
 }
 
-__global__ void __launch_bounds__(RF_BP_THREADS) rf_bp_kernel(rf_bp_params P) {
+// Inputs of one cycle into their planes (wavefront 0).
+__device__ __forceinline__ void bp_inputs(uint64_t* L, const rf_bp_params& P, uint64_t sid, uint32_t cyc,
+                                          uint32_t t) {
+    const uint32_t* ic = P.prog + P.off_in;
+    for (uint32_t k = 0; k < P.n_in; ++k, ic += 3) {
+        const uint64_t r = rf_stim(P.a.seed, sid, (uint64_t)cyc, ic[0]);
+        bp_to_planes(L, ic[2], r, ic[1], t);
+    }
+}
+
+// Fold this cycle's outputs into the lane's hash (wavefront 1).
+__device__ __forceinline__ uint64_t bp_hash(const uint64_t* L, const rf_bp_params& P, uint64_t h,
+                                            uint32_t t) {
+    const uint32_t* hp = P.prog + P.off_hash;
+    for (uint32_t k = 0; k < P.n_hash; ++k) {
+        const uint32_t nbytes = hp[0], width = hp[1];
+        const uint32_t* refs = hp + 2;
+        if (width <= 64) {
+            h = rf_fnv_bytes(h, bp_gather(L, refs, width, t), (int)nbytes);
+        } else {
+            for (uint32_t w = 0; 4u * w < nbytes; ++w) {
+                const uint32_t nb = min(32u, width - 32u * w);
+                h = rf_fnv_bytes(h, bp_gather(L, refs + 32u * w, nb, t), 4);
+            }
+        }
+        hp += 2 + width;
+    }
+    return h;
+}
+
+// Workgroup of NT threads (NT >= 128) per 64 lanes. Lane t of the group is
+// thread t of wavefront 0 (stimulus generation, state load, write-back) and
+// thread 64 + t of wavefront 1 (output hash). After phase B of cycle n,
+// wavefront 1 hashes cycle n while wavefront 0 generates cycle n + 1.
+template <int NT>
+__global__ void __launch_bounds__(NT) rf_bp_kernel(rf_bp_params P) {
     extern __shared__ uint64_t L[];
     __shared__ uint64_t s_act;
     const rf_args& a = P.a;
     const uint32_t tid = threadIdx.x;
     const uint32_t t = tid & 63u;
     const bool w0 = tid < 64u;
+    const bool w1 = tid >= 64u && tid < 128u;
     const uint32_t lane = blockIdx.x * 64u + t;
     const bool inb = lane < a.nlanes;
     const uint32_t* prog = P.prog;
-    bool act = false;
+    const bool act = inb && a.done[lane] == 0;
     uint64_t h = 0, sid = 0;
     uint32_t cyc = 0;
+    if ((w0 || w1) && inb) {
+        h = a.hash[lane];
+        cyc = a.cyc[lane];
+        sid = a.stim[lane];
+    }
+    constexpr uint32_t NW = NT / 64;
+    const uint32_t wave = tid >> 6;
     if (w0) {
-        if (inb) {
-            act = a.done[lane] == 0;
-            h = a.hash[lane];
-            cyc = a.cyc[lane];
-            sid = a.stim[lane];
-        }
         const uint64_t am = __ballot((int)act);
         if (t == 0) {
             s_act = am;
             L[0] = 0;
         }
-        const uint32_t* sl = prog + P.off_state;
-        for (uint32_t k = 0; k < P.n_state; ++k, sl += 6) {
+    }
+    // state planes from the lane-minor arrays, spread over all wavefronts
+    // (lane t of the group is thread t of every wavefront): single-bit
+    // variables by one ballot each, wider ones by transposes
+    {
+        const uint32_t* st = prog + P.off_state;
+        // batches of 8 single-bit entries, round-robin over the wavefronts
+        for (uint32_t k0 = wave * 8u; k0 < P.n_state1; k0 += NW * 8u) {
+            const uint32_t cnt = min(8u, P.n_state1 - k0);
+            uint64_t v[8];
+#pragma unroll
+            for (uint32_t j = 0; j < 8u; ++j) {
+                v[j] = 0;
+                if (j < cnt && inb) {
+                    const uint32_t* e = st + 6u * (k0 + j);
+                    v[j] = bp_load(a, e[0], (uint64_t)e[1] | ((uint64_t)e[2] << 32), lane);
+                }
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < 8u; ++j) {
+                if (j < cnt) {
+                    const uint64_t pl = __ballot((int)(v[j] & 1u));
+                    if (t == j) L[st[6u * (k0 + j) + 5]] = pl;
+                }
+            }
+        }
+        for (uint32_t k2 = P.n_state1 + wave; k2 < P.n_state; k2 += NW) {
+            const uint32_t* sl = st + 6u * k2;
             const uint32_t cls = sl[0], words = sl[3], width = sl[4], s0 = sl[5];
             const uint64_t loc = (uint64_t)sl[1] | ((uint64_t)sl[2] << 32);
             if (width <= 64) {
@@ -141,6 +224,8 @@ __global__ void __launch_bounds__(RF_BP_THREADS) rf_bp_kernel(rf_bp_params P) {
             }
         }
     }
+    if (w0 && a.ncycles) bp_inputs(L, P, sid, cyc, t);
+    __syncthreads();
     const uint32_t* lev[3];
     const uint2* gates[3];
     for (int k = 0; k < 3; ++k) {
@@ -148,63 +233,56 @@ __global__ void __launch_bounds__(RF_BP_THREADS) rf_bp_kernel(rf_bp_params P) {
         gates[k] = reinterpret_cast<const uint2*>(prog + P.off_gates[k]);
     }
     const uint32_t* commit = prog + P.off_commit;
+    const uint64_t am = s_act;
     for (uint32_t n = 0; n < a.ncycles; ++n) {
-        if (w0) {
-            const uint32_t* ic = prog + P.off_in;
-            for (uint32_t k = 0; k < P.n_in; ++k, ic += 3) {
-                const uint64_t r = rf_stim(a.seed, sid, (uint64_t)(cyc + n), ic[0]);
-                bp_to_planes(L, ic[2], r, ic[1], t);
-            }
+        bp_phase<NT>(L, lev[0], gates[0], P.nlev[0], tid);
+        for (uint32_t i = tid; i < P.n_commit; i += NT) {
+            const uint32_t st = commit[2 * i], dn = commit[2 * i + 1];
+            L[st] = (L[dn] & am) | (L[st] & ~am);
         }
         __syncthreads();
-        bp_phase(L, lev[0], gates[0], P.nlev[0], tid);
-        {
-            const uint64_t am = s_act;
-            for (uint32_t i = tid; i < P.n_commit; i += RF_BP_THREADS) {
-                const uint32_t st = commit[2 * i], dn = commit[2 * i + 1];
-                L[st] = (L[dn] & am) | (L[st] & ~am);
-            }
-        }
-        __syncthreads();
-        bp_phase(L, lev[1], gates[1], P.nlev[1], tid);
-        if (w0) {
-            const uint32_t* hp = prog + P.off_hash;
-            uint64_t hn = h;
-            for (uint32_t k = 0; k < P.n_hash; ++k) {
-                const uint32_t nbytes = hp[0], width = hp[1];
-                const uint32_t* refs = hp + 2;
-                if (width <= 64) {
-                    hn = rf_fnv_bytes(hn, bp_gather(L, refs, width, t), (int)nbytes);
-                } else {
-                    for (uint32_t w = 0; 4u * w < nbytes; ++w) {
-                        const uint32_t nb = min(32u, width - 32u * w);
-                        hn = rf_fnv_bytes(hn, bp_gather(L, refs + 32u * w, nb, t), 4);
-                    }
-                }
-                hp += 2 + width;
-            }
+        bp_phase<NT>(L, lev[1], gates[1], P.nlev[1], tid);
+        if (w1) {
+            const uint64_t hn = bp_hash(L, P, h, t);
             if (act) h = hn;
+        } else if (w0 && n + 1 < a.ncycles) {
+            bp_inputs(L, P, sid, cyc + n + 1, t);
         }
+        __syncthreads();
     }
     // settled state after the last cycle, written back to the lane-minor arrays
-    __syncthreads();
-    bp_phase(L, lev[2], gates[2], P.nlev[2], tid);
-    if (w0 && act) {
+    bp_phase<NT>(L, lev[2], gates[2], P.nlev[2], tid);
+    // write-back, spread over all wavefronts: single-bit variables by one
+    // broadcast read each, wider ones by transposes (every lane of a
+    // wavefront takes part; only active lanes store)
+    {
         const uint32_t* wp = prog + P.off_wb;
-        for (uint32_t k = 0; k < P.n_wb; ++k) {
+        for (uint32_t k = wave; k < P.n_wb1; k += NW) {
+            const uint32_t* e = wp + 6u * k;
+            const uint64_t v = (bp_ref(L, e[5]) >> t) & 1ull;
+            if (act) bp_store(a, e[0], (uint64_t)e[1] | ((uint64_t)e[2] << 32), lane, v);
+        }
+        wp += 6u * P.n_wb1;
+        for (uint32_t k = P.n_wb1; k < P.n_wb; ++k) {
             const uint32_t cls = wp[0], words = wp[3], width = wp[4];
-            const uint64_t loc = (uint64_t)wp[1] | ((uint64_t)wp[2] << 32);
-            const uint32_t* refs = wp + 5;
-            if (width <= 64) {
-                bp_store(a, cls, loc, lane, bp_gather(L, refs, width, t));
-            } else {
-                for (uint32_t w = 0; w < words; ++w) {
-                    const uint32_t nb = min(32u, width - 32u * w);
-                    a.i[(size_t)(loc + w) * a.npad + lane] = (uint32_t)bp_gather(L, refs + 32u * w, nb, t);
+            if ((k - P.n_wb1) % NW == wave) {
+                const uint64_t loc = (uint64_t)wp[1] | ((uint64_t)wp[2] << 32);
+                const uint32_t* refs = wp + 5;
+                if (width <= 64) {
+                    const uint64_t v = bp_gather(L, refs, width, t);
+                    if (act) bp_store(a, cls, loc, lane, v);
+                } else {
+                    for (uint32_t w = 0; w < words; ++w) {
+                        const uint32_t nb = min(32u, width - 32u * w);
+                        const uint64_t v = bp_gather(L, refs + 32u * w, nb, t);
+                        if (act) a.i[(size_t)(loc + w) * a.npad + lane] = (uint32_t)v;
+                    }
                 }
             }
             wp += 5 + width;
         }
+    }
+    if (w1 && act) {
         a.hash[lane] = h;
         a.cyc[lane] = cyc + a.ncycles;
     }
@@ -224,6 +302,12 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
     p->n_hash = b[6];
     p->n_wb = b[7];
     for (int k = 0; k < 3; ++k) p->nlev[k] = b[8 + k];
+    p->n_state1 = b[15];
+    p->n_wb1 = b[16];
+    if (p->n_state1 > p->n_state || p->n_wb1 > p->n_wb) {
+        *why = "bit-plane single-bit counts exceed their lists";
+        return -1;
+    }
     const uint32_t ngates[3] = {b[11], b[12], b[13]};
     if (b[14] != words) {
         *why = "bit-plane program size mismatch";
@@ -294,6 +378,10 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
     }
     for (uint32_t k = 0; k < p->n_state; ++k) {
         const uint32_t* sl = b + p->off_state + 6 * k;
+        if ((k < p->n_state1) != (sl[4] == 1)) {
+            *why = "bit-plane state list not ordered single-bit first";
+            return -1;
+        }
         if (sl[0] > 3 || sl[4] == 0 || sl[5] + sl[4] > p->nslots ||
             (sl[4] > 64 && (sl[0] != RF_CLS_I || sl[3] * 32u < sl[4] || sl[3] * 32u >= sl[4] + 32u))) {
             *why = "bit-plane state entry out of range";
@@ -325,6 +413,10 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
     q = p->off_wb;
     for (uint32_t k = 0; k < p->n_wb; ++k) {
         const uint32_t cls = b[q], words_ = b[q + 3], width = b[q + 4];
+        if ((k < p->n_wb1) != (width == 1)) {
+            *why = "bit-plane write-back list not ordered single-bit first";
+            return -1;
+        }
         if (cls > 3 || width == 0 || (width > 64 && (cls != RF_CLS_I || words_ * 32u < width ||
                                                       words_ * 32u >= width + 32u))) {
             *why = "bit-plane write-back entry malformed";
@@ -340,16 +432,27 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
     return 0;
 }
 
-hipError_t rf_bp_launch(const rf_bp_params& p, hipStream_t stream) {
+template <int NT>
+static hipError_t bp_launch_nt(const rf_bp_params& p, hipStream_t stream) {
     const size_t shmem = (size_t)p.nslots * 8u;
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rf_bp_kernel),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rf_bp_kernel<NT>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     const uint32_t grid = (p.a.nlanes + 63u) / 64u;
-    hipLaunchKernelGGL(rf_bp_kernel, dim3(grid), dim3(RF_BP_THREADS), shmem, stream, p);
+    hipLaunchKernelGGL(rf_bp_kernel<NT>, dim3(grid), dim3(NT), shmem, stream, p);
     return hipGetLastError();
+}
+
+hipError_t rf_bp_launch(const rf_bp_params& p, uint32_t threads, hipStream_t stream) {
+    switch (threads) {
+        case 128: return bp_launch_nt<128>(p, stream);
+        case 256: return bp_launch_nt<256>(p, stream);
+        case 512: return bp_launch_nt<512>(p, stream);
+        case 1024: return bp_launch_nt<1024>(p, stream);
+        default: return hipErrorInvalidValue;
+    }
 }

@@ -242,15 +242,21 @@ int rf_create(const rf_design* d, uint32_t n_lanes, const rf_config* cfg, rf_mod
     if (d->abi != RF_DESIGN_ABI)
         return fail(RF_E_DESIGN, "design ABI %u, runtime expects %u", d->abi, RF_DESIGN_ABI);
     if (n_lanes == 0) return fail(RF_E_ARG, "rf_create: n_lanes must be > 0");
+    if (cfg && cfg->bitplane_threads &&
+        !(cfg->bitplane_threads == 128 || cfg->bitplane_threads == 256 || cfg->bitplane_threads == 512 ||
+          cfg->bitplane_threads == 1024))
+        return fail(RF_E_ARG, "bitplane_threads must be 128, 256, 512 or 1024 (got %u)", cfg->bitplane_threads);
     rf_model* m = new rf_model();
     m->d = d;
     m->cfg.settle_unroll = 2;
     m->cfg.converge_limit = 100;
     m->cfg.fused_chunk = 512;
+    m->cfg.bitplane_threads = RF_BP_DEFAULT_THREADS;
     if (cfg) {
         if (cfg->settle_unroll) m->cfg.settle_unroll = cfg->settle_unroll;
         if (cfg->converge_limit) m->cfg.converge_limit = cfg->converge_limit;
         if (cfg->fused_chunk) m->cfg.fused_chunk = cfg->fused_chunk;
+        if (cfg->bitplane_threads) m->cfg.bitplane_threads = cfg->bitplane_threads;
     }
     m->nlanes = n_lanes;
     const uint64_t q = d->block > 64 ? d->block : 64;
@@ -556,7 +562,7 @@ int rf_run_cycles(rf_model* m, uint32_t n, uint32_t mode) {
             p.a = args_for(m, RF_PHASE_EVAL);
             p.a.ncycles = chunk;
             RF_HIP(hipEventRecord(m->ev0, m->stream));
-            RF_HIP(rf_bp_launch(p, m->stream));
+            RF_HIP(rf_bp_launch(p, m->cfg.bitplane_threads, m->stream));
             RF_HIP(hipEventRecord(m->ev1, m->stream));
             RF_HIP(hipEventSynchronize(m->ev1));
             float ms = 0;

@@ -61,7 +61,7 @@ def emit_persistent(comp) -> str:
     gen = Gen(comp)
     gen.top_lines = []
     gen.local = _fused_locals(comp)
-    plan_packing(comp, gen, comp.persistent)
+    plan_packing(comp, gen, comp.stored)
     for v in sorted(gen.local, key=lambda x: x.id):
         gen.declare(v)
     lay = comp.layout

@@ -121,3 +121,27 @@ class Program:
                             h = fnv_bytes(h, from_planes(planes[w0:w0 + 32]), 4)
             out[g0:g0 + n] = h[:n]
         return out
+
+
+def run_state(prog: "Program", seed, nlanes, ncycles, lane):
+    """Run like ``Program.run`` and return the write-back values of one lane
+    as {(cls, loc): value}."""
+    g0 = (lane // 64) * 64
+    n = min(64, nlanes - g0)
+    act = np.uint64((1 << n) - 1) if n < 64 else M64
+    sids = np.arange(g0, g0 + 64, dtype=np.uint64)
+    L = np.zeros(prog.nslots, np.uint64)
+    for c in range(ncycles):
+        for k, nb, s0 in prog.inputs:
+            L[s0:s0 + nb] = to_planes(stim(seed, sids, c, int(k)), int(nb))
+        prog.run_phase(L, 0)
+        st, dn = prog.commit[:, 0], prog.commit[:, 1]
+        L[st] = (L[dn] & act) | (L[st] & ~act)
+        prog.run_phase(L, 1)
+    prog.run_phase(L, 2)
+    t = lane - g0
+    out = {}
+    for cls, loc, words, width, refs in prog.wb:
+        bits = [int((prog.ref(L, int(r)) >> np.uint64(t)) & np.uint64(1)) for r in refs]
+        out[(cls, loc)] = sum(b << i for i, b in enumerate(bits))
+    return out
