@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--design", default="net2")
     ap.add_argument("--forms", default="bitplane,fused,graph")
     ap.add_argument("--chunk", type=int, default=512, help="cycles per fused / bit-plane launch")
+    ap.add_argument("--learned-cycles", type=int, default=400,
+                    help="cycles of the learned-vs-heuristic partition comparison (0: skip)")
     ap.add_argument("--bp-threads", type=int, default=0,
                     help="bit-plane form: threads per 64-lane workgroup (0: runtime default)")
     ap.add_argument("--workload", default="netlist", choices=["netlist", "alu", "systolic", "rv32i"],
@@ -132,6 +134,33 @@ def main():
         log(f"[bench] form={form} {dt:.3f}s {stim_cycles / dt:.4g} stim-cycles/s oracle_match={ok}")
         forms[form] = {"seconds": dt, "value": stim_cycles / dt, "bit_exact_sample": ok,
                        "checked_stimuli": nchk, "stats": st}
+
+    # learned partition: the plain graph form of the same design partitioned
+    # with the cost model's predicted per-vertex costs, against the
+    # instruction-count heuristic, on the same stimuli; kept only if faster
+    # and bit-exact
+    learned = None
+    lname = args.design + "_learned"
+    if args.learned_cycles and lname in registry.load_index():
+        lcd = registry.get(lname)
+        lc = args.learned_cycles
+        lsteps = max(1, lc // cps)
+        res = {}
+        for tag, c in (("heuristic", cd), ("learned", lcd)):
+            log(f"[bench] partition={tag} graph form, {lanes} x {lsteps * cps} cycles, "
+                f"{c.manifest['mtasks']} mtasks")
+            dt, _, h = run_form(c, lanes, MODE_GRAPH, lsteps, cps, args.warmup, seed, args.chunk)
+            res[tag] = (lanes * lsteps * cps / dt, h, c.manifest["mtasks"])
+        same = bool((res["heuristic"][1] == res["learned"][1]).all())
+        ref = oracle.netlist_run(registry.table_bytes(args.design), seed, nstim=min(16, lanes),
+                                 cycles=lsteps * cps, threads=16)
+        exact = same and bool((res["learned"][1][:len(ref)] == ref).all())
+        sp = res["learned"][0] / res["heuristic"][0]
+        learned = {"form": "graph", "cycles": lsteps * cps,
+                   "heuristic": res["heuristic"][0], "learned": res["learned"][0],
+                   "heuristic_mtasks": res["heuristic"][2], "learned_mtasks": res["learned"][2],
+                   "speedup": sp, "bit_exact": exact, "kept": bool(exact and sp > 1.0)}
+        log(f"[bench] learned/heuristic graph throughput {sp:.3f} (bit-exact {exact})")
 
     # the reported form: the fastest one whose sample is bit-exact
     exact = {k: v for k, v in forms.items() if v["bit_exact_sample"]}
@@ -193,6 +222,7 @@ def main():
                    "design": args.design, "stimuli": lanes, "cycles": cycles,
                    "cycles_per_step": cps, "form": main_form, "mtasks": man["mtasks"]},
         "forms": {k: {kk: vv for kk, vv in v.items() if kk != "stats"} for k, v in forms.items()},
+        "learned_partition": learned,
         "bit_exact_sample": all(v["bit_exact_sample"] for v in forms.values()),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,

@@ -766,13 +766,13 @@ def _final_bits(comp, lw: Lowerer, v: Var) -> Optional[List[Lit]]:
 # =====================================================================
 
 BP_MAGIC = 0x50424652   # "RFBP"
-BP_VERSION = 2
+BP_VERSION = 3
 HDR_WORDS = 18
 
 
 def gate_word(g: Tuple[int, int, int, int, int, int]) -> int:
     dst, s, xs, ix, ys, iy = g
-    return dst | (s << 15) | (xs << 30) | (ix << 45) | (ys << 46) | (iy << 61)
+    return dst | (s << 16) | (xs << 32) | (ix << 47) | (ys << 48) | (iy << 63)
 
 
 def encode(prog: Program) -> List[int]:
@@ -789,7 +789,7 @@ def encode(prog: Program) -> List[int]:
       commit      n_commit x {state slot, next-state slot}
       phase A/B/F level offsets (nlev + 1 gate indices, padded to even),
                   then the gates as u64 (lo word first):
-                  dst | s << 15 | x << 30 | ix << 45 | y << 46 | iy << 61,
+                  dst | s << 16 | x << 32 | ix << 47 | y << 48 | iy << 63,
                   dst = s ? (x ^ ix) : (y ^ iy) on 64-lane planes
       hash        n_hash x {nbytes, width, width slot refs}
       write-back  n_wb x {cls, loc_lo, loc_hi, words, width, width slot refs}

@@ -130,7 +130,7 @@ def build_testbench(lib: Path, manifest: dict) -> Path:
     rt = build_runtime()
     rel = os.path.relpath(rt.parent, d)
     _run(["g++", "-O2", "-std=c++17", "-I", str(INCLUDE), "-I", str(CSRC), "-I", str(d), str(tb),
-          "-o", str(exe), "-L", str(d), f"-lV{nm}", "-L", str(rt.parent), "-lrf_verilated",
+          "-o", str(exe), "-L", str(d), f"-l:{lib.name}", "-L", str(rt.parent), "-lrf_verilated",
           "-Wl,-rpath-link,/opt/rocm/lib",
           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/" + rel], log=d / "tb.log")
     return exe

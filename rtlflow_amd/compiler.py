@@ -93,8 +93,10 @@ class Compiled:
         self._storage()
         self._layout()
         # inline form (all logic in one function, state in registers) for
-        # small designs; shared non-inlined mtask bodies for large ones
-        self.calls = sum(self.vcost) > self.opts.inline_limit
+        # small designs; shared non-inlined mtask bodies for large ones (by
+        # instruction count, whatever cost function the partitioner used)
+        instr = sum(process_cost(p) for vx in sch.vertices for p in vx.procs)
+        self.calls = instr > self.opts.inline_limit
         self._bitplane()
 
     def _bitplane(self):

@@ -12,7 +12,7 @@
 
 #define RF_BP_DEFAULT_THREADS 512
 #define RF_BP_MAGIC 0x50424652u
-#define RF_BP_VERSION 2u
+#define RF_BP_VERSION 3u
 #define RF_BP_HDR_WORDS 18u
 
 struct rf_bp_params {

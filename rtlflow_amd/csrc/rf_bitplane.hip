@@ -59,17 +59,21 @@ __device__ __forceinline__ uint64_t bp_gather(const uint64_t* L, const uint32_t*
     return bp_transpose64(row, t);
 }
 
+// Gate record: x = dst | s << 16; y = xs | ix << 15 | ys << 16 | iy << 31.
 __device__ __forceinline__ void bp_gate(uint64_t* L, uint2 w) {
-    const uint32_t dst = w.x & 0x7fffu;
-    const uint32_t s = (w.x >> 15) & 0x7fffu;
-    const uint32_t xs = ((w.x >> 30) | (w.y << 2)) & 0x7fffu;
-    const uint32_t ys = (w.y >> 14) & 0x7fffu;
-    const uint64_t ix = 0ull - (uint64_t)((w.y >> 13) & 1u);
-    const uint64_t iy = 0ull - (uint64_t)((w.y >> 29) & 1u);
+    const uint32_t dst = w.x & 0xffffu;
+    const uint32_t s = w.x >> 16;
+    const uint32_t xs = w.y & 0x7fffu;
+    const uint32_t ys = (w.y >> 16) & 0x7fffu;
+    const uint32_t ix = (uint32_t)((int32_t)(w.y << 16) >> 31);   // 0 or ~0
+    const uint32_t iy = (uint32_t)((int32_t)w.y >> 31);
     const uint64_t S = L[s];
-    const uint64_t X = L[xs] ^ ix;
-    const uint64_t Y = L[ys] ^ iy;
-    L[dst] = (S & X) | (~S & Y);
+    const uint64_t X = L[xs];
+    const uint64_t Y = L[ys];
+    const uint32_t xl = (uint32_t)X ^ ix, xh = (uint32_t)(X >> 32) ^ ix;
+    const uint32_t yl = (uint32_t)Y ^ iy, yh = (uint32_t)(Y >> 32) ^ iy;
+    const uint32_t sl = (uint32_t)S, sh = (uint32_t)(S >> 32);
+    L[dst] = ((uint64_t)((sh & xh) | (~sh & yh)) << 32) | (uint64_t)((sl & xl) | (~sl & yl));
 }
 
 // One phase: level l's gates are spread over the workgroup, a barrier after
@@ -360,8 +364,8 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
         const uint32_t* g = b + p->off_gates[k];
         for (uint32_t i = 0; i < ngates[k]; ++i) {
             const uint64_t w = (uint64_t)g[2 * i] | ((uint64_t)g[2 * i + 1] << 32);
-            const uint32_t f[4] = {(uint32_t)(w & 0x7fff), (uint32_t)((w >> 15) & 0x7fff),
-                                   (uint32_t)((w >> 30) & 0x7fff), (uint32_t)((w >> 46) & 0x7fff)};
+            const uint32_t f[4] = {(uint32_t)(w & 0xffff), (uint32_t)((w >> 16) & 0xffff),
+                                   (uint32_t)((w >> 32) & 0x7fff), (uint32_t)((w >> 48) & 0x7fff)};
             for (uint32_t x : f)
                 if (x >= p->nslots) {
                     *why = "bit-plane gate slot out of range";

@@ -83,12 +83,21 @@ def _file_text(path: Path) -> dict:
     return {}
 
 
+COSTTABLES = Path(__file__).resolve().parent.parent / "costtables"
+
+
 def build_entry(name: str, jobs: int = 8) -> dict:
     from .. import compile_design
     e = ENTRIES[name]
     t0 = time.time()
     spec = e.make()
-    cd = compile_design(spec["verilog"], top=spec["top"], name=name, jobs=jobs, **e.opts)
+    opts = dict(e.opts)
+    table = opts.pop("cost_table", None)
+    if table:
+        # partition with the learned per-vertex costs instead of instruction counts
+        from ..costmodel import TableCost
+        opts["cost_fn"] = TableCost(json.loads((COSTTABLES / f"{table}.json").read_text()))
+    cd = compile_design(spec["verilog"], top=spec["top"], name=name, jobs=jobs, **opts)
     rec = {"name": name, "lib": str(cd.lib_path.relative_to(BUILD)), "opts": e.opts,
            "build_s": round(time.time() - t0, 1)}
     from ..build import build_testbench
@@ -157,6 +166,13 @@ def _fuzz(seed: int):
         return {"verilog": verilog(seed, name=f"fuzz{seed}"), "top": f"fuzz{seed}"}
     return make
 
+
+# the same designs partitioned with the learned cost model's per-vertex costs
+# (tables predicted by the MLP trained on measured mtask kernel times)
+register("net2_learned", _netlist(seed=SEED_CFG[2], name="net2"), threads=4, cost_table="net2",
+         bitplane=False)
+register("systolic_learned", _systolic(), threads=2, cost_table="systolic", bitplane=False)
+register("rv32i_learned", _rv32i(), threads=2, cost_table="rv32i", bitplane=False)
 
 # random operator-coverage designs (compared with the IR interpreter)
 for _s in (1, 2, 3, 4):

@@ -66,7 +66,7 @@ class Program:
             g = b[s["gates" + nm]:s["gates" + nm] + 2 * self.ngates[k]].reshape(-1, 2)
             w = g[:, 0] | (g[:, 1] << np.uint64(32))
             f = lambda sh, bits: ((w >> np.uint64(sh)) & np.uint64((1 << bits) - 1)).astype(np.int64)
-            gates = (f(0, 15), f(15, 15), f(30, 15), f(45, 1), f(46, 15), f(61, 1))
+            gates = (f(0, 16), f(16, 16), f(32, 15), f(47, 1), f(48, 15), f(63, 1))
             self.phases.append((offs, gates))
         o = s["hash"]
         self.hash = []
