@@ -712,22 +712,41 @@ def compile_program(comp) -> Program:
         v = p["var"]
         nbytes = CLS_SIZE[class_of(v)] * (v.slots if v.width > 64 else 1)
         hash_ports.append((nbytes, v.width, [_lit_ref(b, slotB) for b in hbits[v]]))
-    # phase F: every stored variable after the last cycle
+    # phase F: every stored variable after the last cycle, in chunks whose
+    # values all stay live until their write-back; chunks are sized so that
+    # phase F needs no more slots than phases A and B
     wb = []
-    froots = []
     for v in d.vars:
         if v.is_mem or not lay.has(v):
             continue
         bits = _final_bits(comp, lw, v)
-        if bits is None:
-            continue
-        wb.append((v, bits))
-        froots += bits
-    phF, slotF, peakF = build_phase(net, froots, fixed, nfixed)
-    writeback = [(CLS_ID[class_of(v)], lay.loc[v], v.slots, v.width, [_lit_ref(b, slotF) for b in bits])
-                 for v, bits in wb]
-    # single-bit entries first: fixed-size records the kernel indexes directly
-    writeback.sort(key=lambda e: e[3] != 1)
+        if bits is not None:
+            wb.append((v, bits))
+    target = max(peakA, peakB)
+    k = max(1, sum(len(b) for _, b in wb))
+    while True:
+        groups, cur, n = [], [], 0
+        for item in wb:
+            if cur and n + len(item[1]) > k:
+                groups.append(cur)
+                cur, n = [], 0
+            cur.append(item)
+            n += len(item[1])
+        if cur:
+            groups.append(cur)
+        fchunks = []
+        peakF = 0
+        for g in groups:
+            ph, sl, pk = build_phase(net, [b for _, bits in g for b in bits], fixed, nfixed)
+            entries = [(CLS_ID[class_of(v)], lay.loc[v], v.slots, v.width, [_lit_ref(b, sl) for b in bits])
+                       for v, bits in g]
+            # single-bit entries first: fixed-size records the kernel indexes directly
+            entries.sort(key=lambda e: e[3] != 1)
+            fchunks.append((ph, entries))
+            peakF = max(peakF, pk)
+        if peakF <= target or k <= 64:
+            break
+        k //= 2
     state_load.sort(key=lambda e: e[3] != 1)
     prog = Program()
     prog.nslots = max(peakA, peakB, peakF)
@@ -735,13 +754,14 @@ def compile_program(comp) -> Program:
         raise Unsupported(f"{prog.nslots} live bit slots exceed LDS")
     prog.in_chunks = in_chunks
     prog.state_load = state_load
-    prog.phases = [phA, phB, phF]
+    prog.phases = [phA, phB]
+    prog.fchunks = fchunks
     prog.commit = commit
     prog.hash_ports = hash_ports
-    prog.writeback = writeback
     prog.nodes = len(net.kind)
-    prog.gates = [sum(len(l) for l in ph.levels) for ph in prog.phases]
-    prog.nlevels = [len(ph.levels) for ph in prog.phases]
+    prog.gates = [sum(len(l) for l in ph.levels) for ph in prog.phases] + \
+        [sum(len(l) for ph, _ in fchunks for l in ph.levels)]
+    prog.nlevels = [len(ph.levels) for ph in prog.phases] + [sum(len(ph.levels) for ph, _ in fchunks)]
     return prog
 
 
@@ -766,8 +786,9 @@ def _final_bits(comp, lw: Lowerer, v: Var) -> Optional[List[Lit]]:
 # =====================================================================
 
 BP_MAGIC = 0x50424652   # "RFBP"
-BP_VERSION = 3
+BP_VERSION = 4
 HDR_WORDS = 18
+FCHUNK_HDR = 6
 
 
 def gate_word(g: Tuple[int, int, int, int, int, int]) -> int:
@@ -775,87 +796,103 @@ def gate_word(g: Tuple[int, int, int, int, int, int]) -> int:
     return dst | (s << 16) | (xs << 32) | (ix << 47) | (ys << 48) | (iy << 63)
 
 
+def _wb_words(entries) -> List[int]:
+    out: List[int] = []
+    for cls, loc, words, width, refs in entries:
+        out += [cls, loc & 0xffffffff, loc >> 32, words, width] + refs
+    return out
+
+
 def encode(prog: Program) -> List[int]:
     """u32 words of the program blob.
 
     header (18 words): magic, version, nslots, n_in, n_state, n_commit,
-    n_hash, n_wb, nlev[A, B, F], ngates[A, B, F], total words, n_state1,
-    n_wb1, 0 -- the first n_state1 state entries and the first n_wb1
-    write-back entries are single-bit (a write-back record is then 6 words).
-    Then, each section starting at an even word offset:
+    n_hash, n_fchunks, nlev[A, B], ngates[A, B], n_state1, total words, 0 x 4.
+    The first n_state1 state entries are single-bit. Then, each section
+    starting at an even word offset:
       inputs      n_in  x {k, nbits, slot0}: bits of stimulus stream k into
                   slots slot0 .. slot0 + nbits - 1
       state load  n_state x {cls, loc_lo, loc_hi, words, width, slot0}
       commit      n_commit x {state slot, next-state slot}
-      phase A/B/F level offsets (nlev + 1 gate indices, padded to even),
+      phase A, B  level offsets (nlev + 1 gate indices, padded to even),
                   then the gates as u64 (lo word first):
                   dst | s << 16 | x << 32 | ix << 47 | y << 48 | iy << 63,
                   dst = s ? (x ^ ix) : (y ^ iy) on 64-lane planes
       hash        n_hash x {nbytes, width, width slot refs}
-      write-back  n_wb x {cls, loc_lo, loc_hi, words, width, width slot refs}
+      F chunks    n_fchunks x {nlev, ngates, n_wb, n_wb1, words, 0, level
+                  offsets (padded to even), gates, write-back entries
+                  {cls, loc_lo, loc_hi, words, width, width slot refs}, the
+                  first n_wb1 single-bit (6 words each); ``words`` is the
+                  chunk's size including its 6-word header}
     A slot ref is a slot index with the inversion in bit 31.
     """
     out: List[int] = [0] * HDR_WORDS
 
-    def pad():
-        if len(out) & 1:
-            out.append(0)
+    def pad(x):
+        if len(x) & 1:
+            x.append(0)
 
-    pad()
+    def phase_words(ph) -> List[int]:
+        w: List[int] = [0]
+        for lev in ph.levels:
+            w.append(w[-1] + len(lev))
+        pad(w)
+        for lev in ph.levels:
+            for g in lev:
+                gw = gate_word(g)
+                w += [gw & 0xffffffff, gw >> 32]
+        return w
+
+    pad(out)
     for k, nb, s0 in prog.in_chunks:
         out += [k, nb, s0]
-    pad()
+    pad(out)
     for cls, loc, words, width, s0 in prog.state_load:
         out += [cls, loc & 0xffffffff, loc >> 32, words, width, s0]
-    pad()
+    pad(out)
     for st, dn in prog.commit:
         out += [st, dn]
     for ph in prog.phases:
-        pad()
-        offs = [0]
-        for lev in ph.levels:
-            offs.append(offs[-1] + len(lev))
-        out += offs
-        pad()
-        for lev in ph.levels:
-            for g in lev:
-                w = gate_word(g)
-                out += [w & 0xffffffff, w >> 32]
-    pad()
+        pad(out)
+        out += phase_words(ph)
+    pad(out)
     for nbytes, width, refs in prog.hash_ports:
         out += [nbytes, width] + refs
-    pad()
-    for cls, loc, words, width, refs in prog.writeback:
-        out += [cls, loc & 0xffffffff, loc >> 32, words, width] + refs
-    pad()
+    pad(out)
+    for ph, entries in prog.fchunks:
+        body = phase_words(ph) + _wb_words(entries)
+        pad(body)
+        n1 = sum(1 for e in entries if e[3] == 1)
+        out += [len(ph.levels), sum(len(l) for l in ph.levels), len(entries), n1,
+                FCHUNK_HDR + len(body), 0] + body
+    pad(out)
     hdr = [BP_MAGIC, BP_VERSION, prog.nslots, len(prog.in_chunks), len(prog.state_load),
-           len(prog.commit), len(prog.hash_ports), len(prog.writeback)]
+           len(prog.commit), len(prog.hash_ports), len(prog.fchunks)]
     hdr += [len(ph.levels) for ph in prog.phases]
     hdr += [sum(len(l) for l in ph.levels) for ph in prog.phases]
-    hdr += [len(out), sum(1 for e in prog.state_load if e[3] == 1),
-            sum(1 for e in prog.writeback if e[3] == 1), 0]
+    hdr += [sum(1 for e in prog.state_load if e[3] == 1), len(out), 0, 0, 0, 0]
     out[:HDR_WORDS] = hdr
     return out
 
 
-def sections(blob: Sequence[int]) -> Dict[str, int]:
+def sections(blob: Sequence[int]) -> Dict[str, object]:
     """Word offsets of every section of an encoded program (the runtime
     computes the same offsets from the header)."""
     h = blob[:HDR_WORDS]
     assert h[0] == BP_MAGIC and h[1] == BP_VERSION
-    n_in, n_state, n_commit, n_hash, n_wb = h[3:8]
-    nlev = h[8:11]
-    ngates = h[11:14]
+    n_in, n_state, n_commit, n_hash, n_f = h[3:8]
+    nlev = h[8:10]
+    ngates = h[10:12]
     o = HDR_WORDS
     ev = lambda x: x + (x & 1)
-    res = {}
+    res: Dict[str, object] = {}
     res["in"] = o = ev(o)
     o += 3 * n_in
     res["state"] = o = ev(o)
     o += 6 * n_state
     res["commit"] = o = ev(o)
     o += 2 * n_commit
-    for k, name in enumerate("ABF"):
+    for k, name in enumerate("AB"):
         res["lev" + name] = o = ev(o)
         o += nlev[k] + 1
         res["gates" + name] = o = ev(o)
@@ -863,9 +900,16 @@ def sections(blob: Sequence[int]) -> Dict[str, int]:
     res["hash"] = o = ev(o)
     for _ in range(n_hash):
         o += 2 + blob[o + 1]
-    res["wb"] = o = ev(o)
-    for _ in range(n_wb):
-        o += 5 + blob[o + 4]
+    o = ev(o)
+    chunks = []
+    for _ in range(n_f):
+        nl, ng, nwb, nwb1, words = blob[o:o + 5]
+        lev = o + FCHUNK_HDR
+        gates = ev(lev + nl + 1)
+        chunks.append({"at": o, "nlev": nl, "ngates": ng, "n_wb": nwb, "n_wb1": nwb1, "lev": lev,
+                       "gates": gates, "wb": gates + 2 * ng})
+        o += words
+    res["fchunks"] = chunks
     res["end"] = ev(o)
-    assert res["end"] == h[14], (res["end"], h[14])
+    assert res["end"] == h[13], (res["end"], h[13])
     return res

@@ -154,11 +154,33 @@ __device__ __forceinline__ uint64_t bp_hash(const uint64_t* L, const rf_bp_param
     return h;
 }
 
+// Rounds [i0, i1) of the register-resident gate records, a barrier where
+// the round table says a level ends.
+template <int MAXR>
+__device__ __forceinline__ void bp_rounds(uint64_t* L, const uint2 (&d)[MAXR], const rf_bp_params& P,
+                                          int i0, int i1) {
+#pragma unroll
+    for (int i = 0; i < MAXR; ++i) {
+        if (i >= i0 && i < i1) {
+            // an opaque copy: keeps the compiler from hoisting the decoded
+            // fields of every record out of the cycle loop (six registers
+            // per record instead of two)
+            uint2 w = d[i];
+            __asm__ volatile("" : "+v"(w.x), "+v"(w.y));
+            if (w.x != 0xffffffffu) bp_gate(L, w);
+            if ((P.bar >> i) & 1ull) __syncthreads();
+        }
+    }
+}
+
 // Workgroup of NT threads (NT >= 128) per 64 lanes. Lane t of the group is
 // thread t of wavefront 0 (stimulus generation, state load, write-back) and
 // thread 64 + t of wavefront 1 (output hash). After phase B of cycle n,
 // wavefront 1 hashes cycle n while wavefront 0 generates cycle n + 1.
-template <int NT>
+// MAXR > 0: every thread keeps its gate records of phases A and B (at most
+// MAXR rounds) in registers for the whole launch, so the cycle loop issues
+// no global-memory loads; MAXR = 0: records are streamed from memory.
+template <int NT, int MAXR>
 __global__ void __launch_bounds__(NT) rf_bp_kernel(rf_bp_params P) {
     extern __shared__ uint64_t L[];
     __shared__ uint64_t s_act;
@@ -230,22 +252,41 @@ __global__ void __launch_bounds__(NT) rf_bp_kernel(rf_bp_params P) {
     }
     if (w0 && a.ncycles) bp_inputs(L, P, sid, cyc, t);
     __syncthreads();
-    const uint32_t* lev[3];
-    const uint2* gates[3];
-    for (int k = 0; k < 3; ++k) {
+    const uint32_t* lev[2];
+    const uint2* gates[2];
+    for (int k = 0; k < 2; ++k) {
         lev[k] = prog + P.off_lev[k];
         gates[k] = reinterpret_cast<const uint2*>(prog + P.off_gates[k]);
     }
     const uint32_t* commit = prog + P.off_commit;
     const uint64_t am = s_act;
+    constexpr int MR = MAXR > 0 ? MAXR : 1;
+    uint2 d[MR];
+    if constexpr (MAXR > 0) {
+        const uint2* pg = reinterpret_cast<const uint2*>(prog);
+#pragma unroll
+        for (int i = 0; i < MAXR; ++i) {
+            d[i] = make_uint2(0xffffffffu, 0u);
+            if (i < (int)(P.nr_a + P.nr_b)) {
+                const uint32_t idx = P.rb[i] + tid;
+                if (idx < P.re[i]) d[i] = pg[idx];
+            }
+        }
+    }
     for (uint32_t n = 0; n < a.ncycles; ++n) {
-        bp_phase<NT>(L, lev[0], gates[0], P.nlev[0], tid);
+        if constexpr (MAXR > 0)
+            bp_rounds<MR>(L, d, P, 0, (int)P.nr_a);
+        else
+            bp_phase<NT>(L, lev[0], gates[0], P.nlev[0], tid);
         for (uint32_t i = tid; i < P.n_commit; i += NT) {
             const uint32_t st = commit[2 * i], dn = commit[2 * i + 1];
             L[st] = (L[dn] & am) | (L[st] & ~am);
         }
         __syncthreads();
-        bp_phase<NT>(L, lev[1], gates[1], P.nlev[1], tid);
+        if constexpr (MAXR > 0)
+            bp_rounds<MR>(L, d, P, (int)P.nr_a, (int)(P.nr_a + P.nr_b));
+        else
+            bp_phase<NT>(L, lev[1], gates[1], P.nlev[1], tid);
         if (w1) {
             const uint64_t hn = bp_hash(L, P, h, t);
             if (act) h = hn;
@@ -254,22 +295,28 @@ __global__ void __launch_bounds__(NT) rf_bp_kernel(rf_bp_params P) {
         }
         __syncthreads();
     }
-    // settled state after the last cycle, written back to the lane-minor arrays
-    bp_phase<NT>(L, lev[2], gates[2], P.nlev[2], tid);
-    // write-back, spread over all wavefronts: single-bit variables by one
-    // broadcast read each, wider ones by transposes (every lane of a
+    // settled state after the last cycle, written back to the lane-minor
+    // arrays chunk by chunk (a chunk's values stay in LDS until written);
+    // the write-back is spread over all wavefronts: single-bit variables by
+    // one broadcast read each, wider ones by transposes (every lane of a
     // wavefront takes part; only active lanes store)
-    {
-        const uint32_t* wp = prog + P.off_wb;
-        for (uint32_t k = wave; k < P.n_wb1; k += NW) {
+    const uint32_t* fc = prog + P.off_f;
+    for (uint32_t c = 0; c < P.n_f; ++c) {
+        const uint32_t fnl = fc[0], fng = fc[1], nwb = fc[2], nwb1 = fc[3];
+        const uint32_t* flev = fc + RF_BP_FCHUNK_HDR;
+        uint32_t go = RF_BP_FCHUNK_HDR + fnl + 1;
+        go += go & 1u;
+        bp_phase<NT>(L, flev, reinterpret_cast<const uint2*>(fc + go), fnl, tid);
+        const uint32_t* wp = fc + go + 2u * fng;
+        for (uint32_t k = wave; k < nwb1; k += NW) {
             const uint32_t* e = wp + 6u * k;
             const uint64_t v = (bp_ref(L, e[5]) >> t) & 1ull;
             if (act) bp_store(a, e[0], (uint64_t)e[1] | ((uint64_t)e[2] << 32), lane, v);
         }
-        wp += 6u * P.n_wb1;
-        for (uint32_t k = P.n_wb1; k < P.n_wb; ++k) {
+        wp += 6u * nwb1;
+        for (uint32_t k = nwb1; k < nwb; ++k) {
             const uint32_t cls = wp[0], words = wp[3], width = wp[4];
-            if ((k - P.n_wb1) % NW == wave) {
+            if ((k - nwb1) % NW == wave) {
                 const uint64_t loc = (uint64_t)wp[1] | ((uint64_t)wp[2] << 32);
                 const uint32_t* refs = wp + 5;
                 if (width <= 64) {
@@ -285,6 +332,8 @@ __global__ void __launch_bounds__(NT) rf_bp_kernel(rf_bp_params P) {
             }
             wp += 5 + width;
         }
+        __syncthreads();
+        fc += fc[4];
     }
     if (w1 && act) {
         a.hash[lane] = h;
@@ -294,29 +343,50 @@ __global__ void __launch_bounds__(NT) rf_bp_kernel(rf_bp_params P) {
 
 }  // namespace
 
-int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char** why) {
-    if (words < RF_BP_HDR_WORDS || b[0] != RF_BP_MAGIC || b[1] != RF_BP_VERSION) {
-        *why = "bad bit-plane program header";
-        return -1;
+static bool bp_check_phase(const uint32_t* lv, uint32_t nlev, const uint32_t* g, uint32_t ngates,
+                           uint32_t nslots) {
+    if (lv[0] != 0 || lv[nlev] != ngates) return false;
+    for (uint32_t l = 0; l < nlev; ++l)
+        if (lv[l + 1] < lv[l]) return false;
+    for (uint32_t i = 0; i < ngates; ++i) {
+        const uint64_t w = (uint64_t)g[2 * i] | ((uint64_t)g[2 * i + 1] << 32);
+        const uint32_t f[4] = {(uint32_t)(w & 0xffff), (uint32_t)((w >> 16) & 0xffff),
+                               (uint32_t)((w >> 32) & 0x7fff), (uint32_t)((w >> 48) & 0x7fff)};
+        for (uint32_t x : f)
+            if (x >= nslots) return false;
     }
+    return true;
+}
+
+// A write-back / hash value width and byte count that the kernel handles.
+static bool bp_check_refs(const uint32_t* refs, uint32_t width, uint32_t nslots) {
+    for (uint32_t i = 0; i < width; ++i)
+        if ((refs[i] & 0x7fffffffu) >= nslots) return false;
+    return true;
+}
+
+int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char** why) {
+#define BP_FAIL(msg)   \
+    do {               \
+        *why = (msg);  \
+        return -1;     \
+    } while (0)
+    if (words < RF_BP_HDR_WORDS || b[0] != RF_BP_MAGIC || b[1] != RF_BP_VERSION)
+        BP_FAIL("bad bit-plane program header");
     p->nslots = b[2];
     p->n_in = b[3];
     p->n_state = b[4];
     p->n_commit = b[5];
     p->n_hash = b[6];
-    p->n_wb = b[7];
-    for (int k = 0; k < 3; ++k) p->nlev[k] = b[8 + k];
-    p->n_state1 = b[15];
-    p->n_wb1 = b[16];
-    if (p->n_state1 > p->n_state || p->n_wb1 > p->n_wb) {
-        *why = "bit-plane single-bit counts exceed their lists";
-        return -1;
-    }
-    const uint32_t ngates[3] = {b[11], b[12], b[13]};
-    if (b[14] != words) {
-        *why = "bit-plane program size mismatch";
-        return -1;
-    }
+    p->n_f = b[7];
+    p->nlev[0] = b[8];
+    p->nlev[1] = b[9];
+    const uint32_t ngates[2] = {b[10], b[11]};
+    p->n_state1 = b[12];
+    if (b[13] != words) BP_FAIL("bit-plane program size mismatch");
+    if (p->n_state1 > p->n_state) BP_FAIL("bit-plane single-bit count exceeds the state list");
+    if (p->nslots == 0 || p->nslots > 0x7fffu || (uint64_t)p->nslots * 8u > 160u * 1024u - 64u)
+        BP_FAIL("bit-plane program needs more LDS than a workgroup has");
     auto ev = [](uint64_t x) { return x + (x & 1); };
     uint64_t o = ev(RF_BP_HDR_WORDS);
     p->off_in = (uint32_t)o;
@@ -327,7 +397,7 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
     o = ev(o);
     p->off_commit = (uint32_t)o;
     o += 2ull * p->n_commit;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 2; ++k) {
         o = ev(o);
         p->off_lev[k] = (uint32_t)o;
         o += p->nlev[k] + 1ull;
@@ -335,128 +405,122 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
         p->off_gates[k] = (uint32_t)o;
         o += 2ull * ngates[k];
     }
+    if (o > words) BP_FAIL("bit-plane sections exceed the program");
+    for (int k = 0; k < 2; ++k)
+        if (!bp_check_phase(b + p->off_lev[k], p->nlev[k], b + p->off_gates[k], ngates[k], p->nslots))
+            BP_FAIL("bit-plane phase A/B malformed");
     o = ev(o);
     p->off_hash = (uint32_t)o;
-    for (uint32_t k = 0; k < p->n_hash && o + 2 <= words; ++k) o += 2ull + b[o + 1];
+    for (uint32_t k = 0; k < p->n_hash; ++k) {
+        if (o + 2 > words) BP_FAIL("bit-plane hash section truncated");
+        const uint32_t nbytes = b[o], width = b[o + 1];
+        if (width == 0 || o + 2 + width > words) BP_FAIL("bit-plane hash entry malformed");
+        if ((width <= 64 && (nbytes > 8 || nbytes * 8u < width)) ||
+            (width > 64 && (nbytes % 4u || nbytes * 8u < width || nbytes * 8u >= width + 32u)))
+            BP_FAIL("bit-plane hash entry malformed");
+        if (!bp_check_refs(b + o + 2, width, p->nslots)) BP_FAIL("bit-plane hash slot out of range");
+        o += 2ull + width;
+    }
     o = ev(o);
-    p->off_wb = (uint32_t)o;
-    for (uint32_t k = 0; k < p->n_wb && o + 5 <= words; ++k) o += 5ull + b[o + 4];
-    if (ev(o) != words) {
-        *why = "bit-plane program sections do not add up";
-        return -1;
-    }
-    if (p->nslots == 0 || p->nslots > 0x7fffu || (uint64_t)p->nslots * 8u > 160u * 1024u - 64u) {
-        *why = "bit-plane program needs more LDS than a workgroup has";
-        return -1;
-    }
-    // every slot index in the program must be below nslots
-    for (int k = 0; k < 3; ++k) {
-        const uint32_t* lv = b + p->off_lev[k];
-        if (lv[0] != 0 || lv[p->nlev[k]] != ngates[k]) {
-            *why = "bit-plane level table inconsistent";
-            return -1;
+    p->off_f = (uint32_t)o;
+    for (uint32_t c = 0; c < p->n_f; ++c) {
+        if (o + RF_BP_FCHUNK_HDR > words) BP_FAIL("bit-plane write-back chunk truncated");
+        const uint32_t nl = b[o], ng = b[o + 1], nwb = b[o + 2], nwb1 = b[o + 3], cw = b[o + 4];
+        if (nwb1 > nwb || cw < RF_BP_FCHUNK_HDR || o + cw > words) BP_FAIL("bit-plane write-back chunk malformed");
+        uint64_t g = o + RF_BP_FCHUNK_HDR + nl + 1;
+        g = ev(g);
+        uint64_t e = g + 2ull * ng;
+        if (e > o + cw) BP_FAIL("bit-plane write-back chunk malformed");
+        if (!bp_check_phase(b + o + RF_BP_FCHUNK_HDR, nl, b + g, ng, p->nslots))
+            BP_FAIL("bit-plane write-back phase malformed");
+        for (uint32_t k = 0; k < nwb; ++k) {
+            if (e + 5 > o + cw) BP_FAIL("bit-plane write-back entry truncated");
+            const uint32_t cls = b[e], wds = b[e + 3], width = b[e + 4];
+            if ((k < nwb1) != (width == 1)) BP_FAIL("bit-plane write-back list not ordered single-bit first");
+            if (cls > 3 || width == 0 || e + 5 + width > o + cw ||
+                (width > 64 && (cls != RF_CLS_I || wds * 32u < width || wds * 32u >= width + 32u)))
+                BP_FAIL("bit-plane write-back entry malformed");
+            if (!bp_check_refs(b + e + 5, width, p->nslots)) BP_FAIL("bit-plane write-back slot out of range");
+            e += 5ull + width;
         }
-        for (uint32_t l = 0; l < p->nlev[k]; ++l)
-            if (lv[l + 1] < lv[l]) {
-                *why = "bit-plane level table not monotonic";
-                return -1;
-            }
-        const uint32_t* g = b + p->off_gates[k];
-        for (uint32_t i = 0; i < ngates[k]; ++i) {
-            const uint64_t w = (uint64_t)g[2 * i] | ((uint64_t)g[2 * i + 1] << 32);
-            const uint32_t f[4] = {(uint32_t)(w & 0xffff), (uint32_t)((w >> 16) & 0xffff),
-                                   (uint32_t)((w >> 32) & 0x7fff), (uint32_t)((w >> 48) & 0x7fff)};
-            for (uint32_t x : f)
-                if (x >= p->nslots) {
-                    *why = "bit-plane gate slot out of range";
-                    return -1;
-                }
-        }
+        o += cw;
     }
+    if (ev(o) != words) BP_FAIL("bit-plane program sections do not add up");
     for (uint32_t k = 0; k < p->n_in; ++k) {
         const uint32_t* ic = b + p->off_in + 3 * k;
-        if (ic[1] > 64 || ic[2] + ic[1] > p->nslots) {
-            *why = "bit-plane input slot out of range";
-            return -1;
-        }
+        if (ic[1] > 64 || ic[2] + ic[1] > p->nslots) BP_FAIL("bit-plane input slot out of range");
     }
     for (uint32_t k = 0; k < p->n_state; ++k) {
         const uint32_t* sl = b + p->off_state + 6 * k;
-        if ((k < p->n_state1) != (sl[4] == 1)) {
-            *why = "bit-plane state list not ordered single-bit first";
-            return -1;
-        }
+        if ((k < p->n_state1) != (sl[4] == 1)) BP_FAIL("bit-plane state list not ordered single-bit first");
         if (sl[0] > 3 || sl[4] == 0 || sl[5] + sl[4] > p->nslots ||
-            (sl[4] > 64 && (sl[0] != RF_CLS_I || sl[3] * 32u < sl[4] || sl[3] * 32u >= sl[4] + 32u))) {
-            *why = "bit-plane state entry out of range";
-            return -1;
-        }
+            (sl[4] > 64 && (sl[0] != RF_CLS_I || sl[3] * 32u < sl[4] || sl[3] * 32u >= sl[4] + 32u)))
+            BP_FAIL("bit-plane state entry out of range");
     }
     for (uint32_t k = 0; k < p->n_commit; ++k) {
         const uint32_t* c = b + p->off_commit + 2 * k;
-        if (c[0] >= p->nslots || c[1] >= p->nslots) {
-            *why = "bit-plane commit slot out of range";
-            return -1;
-        }
-    }
-    uint64_t q = p->off_hash;
-    for (uint32_t k = 0; k < p->n_hash; ++k) {
-        const uint32_t nbytes = b[q], width = b[q + 1];
-        if (width == 0 || (width <= 64 && (nbytes > 8 || nbytes * 8u < width)) ||
-            (width > 64 && (nbytes % 4u || nbytes * 8u < width || nbytes * 8u >= width + 32u))) {
-            *why = "bit-plane hash entry malformed";
-            return -1;
-        }
-        for (uint32_t i = 0; i < width; ++i)
-            if ((b[q + 2 + i] & 0x7fffffffu) >= p->nslots) {
-                *why = "bit-plane hash slot out of range";
-                return -1;
-            }
-        q += 2 + width;
-    }
-    q = p->off_wb;
-    for (uint32_t k = 0; k < p->n_wb; ++k) {
-        const uint32_t cls = b[q], words_ = b[q + 3], width = b[q + 4];
-        if ((k < p->n_wb1) != (width == 1)) {
-            *why = "bit-plane write-back list not ordered single-bit first";
-            return -1;
-        }
-        if (cls > 3 || width == 0 || (width > 64 && (cls != RF_CLS_I || words_ * 32u < width ||
-                                                      words_ * 32u >= width + 32u))) {
-            *why = "bit-plane write-back entry malformed";
-            return -1;
-        }
-        for (uint32_t i = 0; i < width; ++i)
-            if ((b[q + 5 + i] & 0x7fffffffu) >= p->nslots) {
-                *why = "bit-plane write-back slot out of range";
-                return -1;
-            }
-        q += 5 + width;
+        if (c[0] >= p->nslots || c[1] >= p->nslots) BP_FAIL("bit-plane commit slot out of range");
     }
     return 0;
+#undef BP_FAIL
 }
 
-template <int NT>
-static hipError_t bp_launch_nt(const rf_bp_params& p, hipStream_t stream) {
+void rf_bp_rounds(const uint32_t* b, uint32_t threads, rf_bp_params* p) {
+    p->nr_a = p->nr_b = 0;
+    p->bar = 0;
+    uint32_t r = 0;
+    uint32_t nr[2] = {0, 0};
+    for (int k = 0; k < 2; ++k) {
+        const uint32_t* lv = b + p->off_lev[k];
+        const uint32_t g = p->off_gates[k] / 2;   // uint2 index of the phase's first record
+        for (uint32_t l = 0; l < p->nlev[k]; ++l) {
+            const uint32_t lo = lv[l], hi = lv[l + 1];
+            for (uint32_t s = lo; s < hi; s += threads) {
+                if (r >= RF_BP_MAXR) return;
+                p->rb[r] = g + s;
+                p->re[r] = g + hi;
+                if (s + threads >= hi) p->bar |= 1ull << r;
+                ++r;
+                ++nr[k];
+            }
+        }
+    }
+    p->nr_a = nr[0];
+    p->nr_b = nr[1];
+}
+
+template <int NT, int MAXR>
+static hipError_t bp_launch_k(const rf_bp_params& p, hipStream_t stream) {
     const size_t shmem = (size_t)p.nslots * 8u;
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rf_bp_kernel<NT>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rf_bp_kernel<NT, MAXR>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     const uint32_t grid = (p.a.nlanes + 63u) / 64u;
-    hipLaunchKernelGGL(rf_bp_kernel<NT>, dim3(grid), dim3(NT), shmem, stream, p);
+    hipLaunchKernelGGL((rf_bp_kernel<NT, MAXR>), dim3(grid), dim3(NT), shmem, stream, p);
     return hipGetLastError();
+}
+
+template <int NT>
+static hipError_t bp_launch_nt(const rf_bp_params& p, hipStream_t stream) {
+    const uint32_t r = p.nr_a + p.nr_b;
+    if (p.nr_a == 0 || r > RF_BP_MAXR) return bp_launch_k<NT, 0>(p, stream);
+    if (r <= 16) return bp_launch_k<NT, 16>(p, stream);
+    if (r <= 24) return bp_launch_k<NT, 24>(p, stream);
+    if (r <= 32) return bp_launch_k<NT, 32>(p, stream);
+    if (r <= 40) return bp_launch_k<NT, 40>(p, stream);
+    return bp_launch_k<NT, 48>(p, stream);
 }
 
 hipError_t rf_bp_launch(const rf_bp_params& p, uint32_t threads, hipStream_t stream) {
     switch (threads) {
-        case 128: return bp_launch_nt<128>(p, stream);
+        case 128: return bp_launch_k<128, 0>(p, stream);
         case 256: return bp_launch_nt<256>(p, stream);
         case 512: return bp_launch_nt<512>(p, stream);
-        case 1024: return bp_launch_nt<1024>(p, stream);
+        case 1024: return bp_launch_k<1024, 0>(p, stream);
         default: return hipErrorInvalidValue;
     }
 }

@@ -193,12 +193,20 @@ int load_bitplane(rf_model* m) {
         const uint64_t n = sl[4] > 64 ? sl[3] : 1;
         if (loc + n > d->slots[sl[0]]) return fail(RF_E_DESIGN, "bit-plane state slot outside the layout");
     }
-    uint64_t q = p.off_wb;
-    for (uint32_t k = 0; k < p.n_wb; ++k) {
-        const uint64_t loc = (uint64_t)b[q + 1] | ((uint64_t)b[q + 2] << 32);
-        const uint64_t n = b[q + 4] > 64 ? b[q + 3] : 1;
-        if (loc + n > d->slots[b[q]]) return fail(RF_E_DESIGN, "bit-plane write-back slot outside the layout");
-        q += 5 + b[q + 4];
+    uint64_t q = p.off_f;
+    for (uint32_t c = 0; c < p.n_f; ++c) {
+        const uint32_t nl = b[q], ng = b[q + 1], nwb = b[q + 2];
+        uint64_t e = q + RF_BP_FCHUNK_HDR + nl + 1;
+        e += e & 1;
+        e += 2ull * ng;
+        for (uint32_t k = 0; k < nwb; ++k) {
+            const uint64_t loc = (uint64_t)b[e + 1] | ((uint64_t)b[e + 2] << 32);
+            const uint64_t n = b[e + 4] > 64 ? b[e + 3] : 1;
+            if (loc + n > d->slots[b[e]])
+                return fail(RF_E_DESIGN, "bit-plane write-back slot outside the layout");
+            e += 5 + b[e + 4];
+        }
+        q += b[q + 4];
     }
     const size_t bytes = d->bitplane_words * 4;
     RF_HIP(hipMalloc(&m->bp_prog, bytes));
@@ -559,6 +567,7 @@ int rf_run_cycles(rf_model* m, uint32_t n, uint32_t mode) {
         while (left) {
             const uint32_t chunk = std::min(left, m->cfg.fused_chunk);
             rf_bp_params p = m->bp;
+            rf_bp_rounds(m->d->bitplane, m->cfg.bitplane_threads, &p);
             p.a = args_for(m, RF_PHASE_EVAL);
             p.a.ncycles = chunk;
             RF_HIP(hipEventRecord(m->ev0, m->stream));

@@ -52,37 +52,43 @@ class Program:
         self.b = np.asarray(blob, dtype=np.uint64)
         self.sec = bp.sections(blob)
         h = blob[:bp.HDR_WORDS]
-        self.nslots, self.n_in, self.n_state, self.n_commit, self.n_hash, self.n_wb = h[2:8]
-        self.nlev = h[8:11]
-        self.ngates = h[11:14]
+        self.nslots, self.n_in, self.n_state, self.n_commit, self.n_hash = h[2:7]
+        self.nlev = h[8:10]
+        self.ngates = h[10:12]
         b = self.b
         s = self.sec
         self.inputs = b[s["in"]:s["in"] + 3 * self.n_in].reshape(-1, 3).astype(np.int64)
         self.state = b[s["state"]:s["state"] + 6 * self.n_state].reshape(-1, 6).astype(np.int64)
         self.commit = b[s["commit"]:s["commit"] + 2 * self.n_commit].reshape(-1, 2).astype(np.int64)
-        self.phases = []
-        for k, nm in enumerate("ABF"):
-            offs = b[s["lev" + nm]:s["lev" + nm] + self.nlev[k] + 1].astype(np.int64)
-            g = b[s["gates" + nm]:s["gates" + nm] + 2 * self.ngates[k]].reshape(-1, 2)
-            w = g[:, 0] | (g[:, 1] << np.uint64(32))
-            f = lambda sh, bits: ((w >> np.uint64(sh)) & np.uint64((1 << bits) - 1)).astype(np.int64)
-            gates = (f(0, 16), f(16, 16), f(32, 15), f(47, 1), f(48, 15), f(63, 1))
-            self.phases.append((offs, gates))
+        self.phases = [self._phase(s["lev" + nm], self.nlev[k], s["gates" + nm], self.ngates[k])
+                       for k, nm in enumerate("AB")]
         o = s["hash"]
         self.hash = []
         for _ in range(self.n_hash):
             nbytes, width = int(b[o]), int(b[o + 1])
             self.hash.append((nbytes, width, b[o + 2:o + 2 + width].astype(np.int64)))
             o += 2 + width
-        o = s["wb"]
-        self.wb = []
-        for _ in range(self.n_wb):
-            cls, lo, hi, words, width = (int(x) for x in b[o:o + 5])
-            self.wb.append((cls, lo | (hi << 32), words, width, b[o + 5:o + 5 + width].astype(np.int64)))
-            o += 5 + width
+        self.fchunks = []
+        for c in s["fchunks"]:
+            ph = self._phase(c["lev"], c["nlev"], c["gates"], c["ngates"])
+            o = c["wb"]
+            wb = []
+            for _ in range(c["n_wb"]):
+                cls, lo, hi, words, width = (int(x) for x in b[o:o + 5])
+                wb.append((cls, lo | (hi << 32), words, width, b[o + 5:o + 5 + width].astype(np.int64)))
+                o += 5 + width
+            self.fchunks.append((ph, wb))
+
+    def _phase(self, lev_at, nlev, gates_at, ngates):
+        b = self.b
+        offs = b[lev_at:lev_at + nlev + 1].astype(np.int64)
+        g = b[gates_at:gates_at + 2 * ngates].reshape(-1, 2)
+        w = g[:, 0] | (g[:, 1] << np.uint64(32))
+        f = lambda sh, bits: ((w >> np.uint64(sh)) & np.uint64((1 << bits) - 1)).astype(np.int64)
+        return offs, (f(0, 16), f(16, 16), f(32, 15), f(47, 1), f(48, 15), f(63, 1))
 
     def run_phase(self, L, k):
-        offs, (dst, s, xs, ix, ys, iy) = self.phases[k]
+        offs, (dst, s, xs, ix, ys, iy) = self.phases[k] if isinstance(k, int) else k
         for l in range(len(offs) - 1):
             a, e = offs[l], offs[l + 1]
             S = L[s[a:e]]
@@ -138,10 +144,11 @@ def run_state(prog: "Program", seed, nlanes, ncycles, lane):
         st, dn = prog.commit[:, 0], prog.commit[:, 1]
         L[st] = (L[dn] & act) | (L[st] & ~act)
         prog.run_phase(L, 1)
-    prog.run_phase(L, 2)
     t = lane - g0
     out = {}
-    for cls, loc, words, width, refs in prog.wb:
-        bits = [int((prog.ref(L, int(r)) >> np.uint64(t)) & np.uint64(1)) for r in refs]
-        out[(cls, loc)] = sum(b << i for i, b in enumerate(bits))
+    for ph, wb in prog.fchunks:
+        prog.run_phase(L, ph)
+        for cls, loc, words, width, refs in wb:
+            bits = [int((prog.ref(L, int(r)) >> np.uint64(t)) & np.uint64(1)) for r in refs]
+            out[(cls, loc)] = sum(b << i for i, b in enumerate(bits))
     return out

@@ -101,3 +101,19 @@ def test_supported_random_designs_match_interpreter():
         assert (got == want).all(), (seed, int((got != want).sum()))
         done += 1
     assert done >= 10
+
+
+def test_write_back_chunks_restore_every_stored_value():
+    """Phase F runs in chunks sized to the LDS budget of phases A and B; the
+    written-back values equal the interpreter's settled state."""
+    from bitplane_sim import run_state
+    from lanediff_util import interp_lane_state
+    src, _ = generate(n_gates=1500, n_ff=150, n_pi=32, n_po=32, depth=16, seed=11, name="wbc")
+    comp, blob = _prog(src, threads=4)
+    prog = bp.compile_program(comp)
+    assert len(prog.fchunks) > 1
+    got = run_state(Program(blob), 0x5EED, 70, 13, 66)
+    want = interp_lane_state(elaborate(src), 0x5EED, 66, 13)
+    man = comp.manifest()
+    bad = [v["name"] for v in man["vars"] if v["name"] in want and got.get((v["cls"], v["loc"])) != want[v["name"]]]
+    assert not bad, bad[:5]
