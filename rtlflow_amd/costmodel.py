@@ -371,9 +371,57 @@ class TableCost:
         return self.costs[vx.id]
 
 
+CANDIDATE_THREADS = (2, 3, 4, 6, 8, 12, 16, 24, 32)
+
+
+def partition_candidates(names=TARGETS, threads=CANDIDATE_THREADS, log=print) -> dict:
+    """CPU: partition each target at several granularities (no HIP build) and
+    record every candidate's mtask features, for the learned choice of the
+    partition (``choose_partitions``)."""
+    import json
+    from .compiler import Compiled, Options
+    from .designs import registry
+    from .verilog import elaborate
+    out = {}
+    for name in names:
+        e = registry.ENTRIES[name]
+        spec = e.make()
+        cands = {}
+        for t in threads:
+            opts = {k: v for k, v in e.opts.items() if k not in ("cost_table", "learned_threads")}
+            opts.update(threads=t, bitplane=False)
+            comp = Compiled(elaborate(spec["verilog"], spec["top"]), Options(**opts))
+            fx = FeatureExtractor(comp)
+            cands[str(t)] = [fx.mtask(k) for k in range(len(comp.mtasks))]
+            if log:
+                log(f"candidate {name} threads={t}: {len(comp.mtasks)} mtasks")
+        out[name] = cands
+    d = _cm_dir()
+    d.mkdir(parents=True, exist_ok=True)
+    (d / "candidates.json").write_text(json.dumps(out))
+    return out
+
+
+def choose_partitions(model: Dict, candidates: dict) -> dict:
+    """GPU: the MLP's predicted time of one eval (sum over the candidate's
+    mtask kernels) for every candidate granularity; the choice is the
+    fastest. The instruction-count heuristic cannot make this choice: every
+    candidate has the same total instruction count."""
+    lc = LearnedCost(model)
+    res = {}
+    for name, cands in candidates.items():
+        pred = {t: float(lc.predict_us(f).sum()) for t, f in cands.items()}
+        best = min(pred, key=pred.get)
+        res[name] = {"threads": int(best), "predicted_us": pred}
+    lc.net.close()
+    return res
+
+
 def main(argv=None):
-    """GPU workflow: profile the variants, train, predict the targets' vertex
-    costs. ``python -m rtlflow_amd.costmodel OUTDIR``."""
+    """GPU workflow. ``python -m rtlflow_amd.costmodel OUTDIR`` profiles the
+    variants, trains, and predicts the targets' vertex costs and partition
+    choices; ``python -m rtlflow_amd.costmodel OUTDIR --choose MODEL`` only
+    predicts the partition choices with a trained model."""
     import json
     import sys
     from pathlib import Path
@@ -381,21 +429,30 @@ def main(argv=None):
     argv = argv if argv is not None else sys.argv[1:]
     out = Path(argv[0] if argv else "costmodel_out")
     out.mkdir(parents=True, exist_ok=True)
-    samples = profile_variants(log=lambda *a: print(*a, flush=True))
-    (out / "samples.json").write_text(json.dumps(samples))
-    designs = sorted({s["design"] for s in samples})
-    model = fit(samples)
-    # hold-out check: per design, a model trained without it
-    print(f"trained on {len(samples)} mtasks from {len(designs)} designs: "
-          f"val mse {model['history'][-1][2]:.4f}", flush=True)
-    (out / "model.json").write_text(json.dumps(model))
-    index = json.loads((_cm_dir() / "index.json").read_text())
-    for vname, lib in index.items():
-        rec = json.loads(((BUILD / lib).parent / "features.json").read_text())
-        if "vertex_features" in rec:
-            costs = vertex_costs(model, rec["vertex_features"])
-            (out / f"costs_{rec['design']}.json").write_text(json.dumps(costs))
-            print(f"cost table {rec['design']}: {len(costs)} vertices", flush=True)
+    cand_path = _cm_dir() / "candidates.json"
+    if "--choose" in argv:
+        model = json.loads(Path(argv[argv.index("--choose") + 1]).read_text())
+    else:
+        samples = profile_variants(log=lambda *a: print(*a, flush=True))
+        (out / "samples.json").write_text(json.dumps(samples))
+        designs = sorted({s["design"] for s in samples})
+        model = fit(samples)
+        print(f"trained on {len(samples)} mtasks from {len(designs)} designs: "
+              f"val mse {model['history'][-1][2]:.4f}", flush=True)
+        (out / "model.json").write_text(json.dumps(model))
+        index = json.loads((_cm_dir() / "index.json").read_text())
+        for vname, lib in index.items():
+            rec = json.loads(((BUILD / lib).parent / "features.json").read_text())
+            if "vertex_features" in rec:
+                costs = vertex_costs(model, rec["vertex_features"])
+                (out / f"costs_{rec['design']}.json").write_text(json.dumps(costs))
+                print(f"cost table {rec['design']}: {len(costs)} vertices", flush=True)
+    if cand_path.exists():
+        choice = choose_partitions(model, json.loads(cand_path.read_text()))
+        (out / "partition_choice.json").write_text(json.dumps(choice, indent=1))
+        for name, c in choice.items():
+            print(f"partition choice {name}: threads={c['threads']} "
+                  f"(predicted {c['predicted_us'][str(c['threads'])]:.1f} us per eval)", flush=True)
 
 
 if __name__ == "__main__":

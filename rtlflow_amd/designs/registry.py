@@ -92,6 +92,10 @@ def build_entry(name: str, jobs: int = 8) -> dict:
     t0 = time.time()
     spec = e.make()
     opts = dict(e.opts)
+    learned = opts.pop("learned_threads", None)
+    if learned:
+        choice = json.loads((COSTTABLES / "partition_choice.json").read_text())
+        opts["threads"] = int(choice[learned]["threads"])
     table = opts.pop("cost_table", None)
     if table:
         # partition with the learned per-vertex costs instead of instruction counts
@@ -167,12 +171,14 @@ def _fuzz(seed: int):
     return make
 
 
-# the same designs partitioned with the learned cost model's per-vertex costs
-# (tables predicted by the MLP trained on measured mtask kernel times)
-register("net2_learned", _netlist(seed=SEED_CFG[2], name="net2"), threads=4, cost_table="net2",
+# the same designs partitioned by the learned cost model: the granularity
+# whose predicted eval time (sum of the MLP's predicted mtask kernel times)
+# is least ("learned_threads"); "cost_table" would also feed the MLP's
+# per-vertex costs to the contraction (measured slower on net2)
+register("net2_learned", _netlist(seed=SEED_CFG[2], name="net2"), learned_threads="net2",
          bitplane=False)
-register("systolic_learned", _systolic(), threads=2, cost_table="systolic", bitplane=False)
-register("rv32i_learned", _rv32i(), threads=2, cost_table="rv32i", bitplane=False)
+register("systolic_learned", _systolic(), learned_threads="systolic", bitplane=False)
+register("rv32i_learned", _rv32i(), learned_threads="rv32i", bitplane=False)
 
 # random operator-coverage designs (compared with the IR interpreter)
 for _s in (1, 2, 3, 4):

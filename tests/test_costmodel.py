@@ -42,3 +42,23 @@ def test_mtask_features_add_up():
         assert f[1] <= sum(fx.vertex(v)[1] for v in mt.vids)   # fan-in shrinks on merging
     x = encode([fx.mtask(k) for k in range(len(comp.mtasks))])
     assert x.shape == (len(comp.mtasks), 6) and np.isfinite(x).all()
+
+
+def test_committed_partition_choice_is_the_predicted_minimum():
+    """The learned choice of partition granularity: the committed choice per
+    target is the candidate with the least predicted eval time, a choice the
+    instruction-count heuristic cannot make (every candidate partition of a
+    design has the same total instruction count)."""
+    import json
+    from rtlflow_amd.designs.registry import COSTTABLES
+    choice = json.loads((COSTTABLES / "partition_choice.json").read_text())
+    for name, c in choice.items():
+        pred = {int(t): v for t, v in c["predicted_us"].items()}
+        assert c["threads"] == min(pred, key=pred.get)
+    src, _ = generate(n_gates=800, n_ff=80, n_pi=16, n_po=16, depth=12, seed=4, name="eq")
+    totals = set()
+    for t in (2, 4, 8):
+        comp = Compiled(elaborate(src), Options(threads=t, bitplane=False))
+        fx = FeatureExtractor(comp)
+        totals.add(sum(fx.mtask(k)[0] for k in range(len(comp.mtasks))))
+    assert len(totals) == 1
