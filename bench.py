@@ -113,6 +113,11 @@ def main():
 
     seed = registry.SEED_CFG[2]
     cd = registry.get(args.design)
+    if not args.bp_threads:
+        from rtlflow_amd.autotune import choice
+        tuned = choice(args.design)
+        if tuned and tuned.get("form") == "bitplane":
+            args.bp_threads = tuned["knobs"]["bp_threads"]
     man = cd.manifest
     lanes, cps, steps = args.lanes, args.cycles_per_step, args.steps
     cycles = cps * steps
@@ -255,9 +260,16 @@ def bench_other(args):
         run = oracle.alu_run if wl == "alu" else oracle.systolic_run
         nchk = min(args.check, lanes)
         ref = run(seed, nchk, cycles, threads=16)
-        for form, mode in (("fused", MODE_FUSED), ("graph", MODE_GRAPH)):
+        runs = [("fused", cd, MODE_FUSED, args.chunk), ("graph", cd, MODE_GRAPH, args.chunk)]
+        if f"{wl}_tuned" in registry.load_index():
+            # the autotuner's compile-time choice (stimuli per block, memory layout)
+            from rtlflow_amd.autotune import choice
+            t = choice(wl)
+            runs.append(("tuned", registry.get(f"{wl}_tuned"), MODE_FUSED,
+                         t["knobs"].get("chunk", args.chunk)))
+        for form, c, mode, chunk in runs:
             log(f"[bench] {wl} form={form} lanes={lanes} cycles={cycles}")
-            dt, st, h = run_form(cd, lanes, mode, steps, cps, args.warmup, seed, args.chunk)
+            dt, st, h = run_form(c, lanes, mode, steps, cps, args.warmup, seed, chunk)
             ok = bool((h[:nchk] == ref).all())
             forms[form] = {"seconds": dt, "value": lanes * cycles / dt, "bit_exact_sample": ok,
                            "checked_stimuli": nchk}
@@ -286,6 +298,22 @@ def bench_other(args):
         forms["batched"] = {"seconds": dt, "value": total / dt, "bit_exact_sample": ok,
                             "checked_stimuli": nchk, "lanes": lanes}
         log(f"[bench] batched {dt:.3f}s {total / dt:.4g} stim-cycles/s oracle_match={ok}")
+        if "rv32i_tuned" in registry.load_index():
+            # the autotuner's choice: stimuli per block, memory layout, lanes per
+            # queue (measured on a quarter-size queue, scaled to this one)
+            from rtlflow_amd.autotune import WORKLOADS, choice
+            t = choice("rv32i")
+            tl = t["knobs"]["lanes"] * nstim // WORKLOADS["rv32i"]["nstim"]
+            with registry.get("rv32i_tuned").model(tl) as m:
+                m.set_mem_images("imem", progs)
+                m.sync()
+                t0 = time.perf_counter()
+                out2 = m.submit_queue(seed, budgets)
+                dt2 = time.perf_counter() - t0
+            ok2 = bool((out2 == out).all())
+            forms["tuned"] = {"seconds": dt2, "value": total / dt2, "bit_exact_sample": ok2 and ok,
+                              "checked_stimuli": nchk, "lanes": tl}
+            log(f"[bench] tuned batched {dt2:.3f}s {total / dt2:.4g} stim-cycles/s same_hashes={ok2}")
         med = int(np.median(budgets))
         gl = 4096
         with cd.model(gl) as m:

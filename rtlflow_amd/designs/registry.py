@@ -180,6 +180,24 @@ register("net2_learned", _netlist(seed=SEED_CFG[2], name="net2"), learned_thread
 register("systolic_learned", _systolic(), learned_threads="systolic", bitplane=False)
 register("rv32i_learned", _rv32i(), learned_threads="rv32i", bitplane=False)
 
+# the autotuner's choice of compile-time options, where it measured one
+# (rtlflow_amd/tuned/<name>.json, written by rtlflow_amd.autotune on the GPU)
+def _register_tuned():
+    tuned = Path(__file__).resolve().parent.parent / "tuned"
+    for name in ("alu", "systolic", "rv32i"):
+        p = tuned / f"{name}.json"
+        if not p.exists():
+            continue
+        best = json.loads(p.read_text())["best"]
+        if best.get("compile"):
+            e = ENTRIES[name]
+            opts = dict(e.opts)
+            opts.update(best["compile"])
+            register(f"{name}_tuned", e.make, **opts)
+
+
+_register_tuned()
+
 # random operator-coverage designs (compared with the IR interpreter)
 for _s in (1, 2, 3, 4):
     register(f"fuzz{_s}", _fuzz(_s), threads=3)

@@ -1,0 +1,41 @@
+"""The autotuner's chosen builds (other stimuli per block, memory layout) stay
+bit-exact against the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from rtlflow_amd.designs import registry, rv32i
+from rtlflow_amd.model import MODE_FUSED, MODE_GRAPH
+
+
+def _get(name):
+    if name not in registry.load_index():
+        pytest.skip(f"{name} not built (no autotuner choice)")
+    return registry.get(name)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,run", [("alu", oracle.alu_run), ("systolic", oracle.systolic_run)])
+def test_tuned_protocol(gpu, name, run):
+    cd = _get(f"{name}_tuned")
+    n, cycles, seed = 300, 60, 0x7E57
+    ref = run(seed, n, cycles)
+    for mode in (MODE_FUSED, MODE_GRAPH):
+        with cd.model(n) as m:
+            m.begin(seed)
+            m.run_cycles(cycles, mode)
+            got = m.hashes()
+        assert (got == ref).all(), f"{name}_tuned mode {mode}: {int((got != ref).sum())} differ"
+
+
+@pytest.mark.gpu
+def test_tuned_rv32i_queue(gpu):
+    cd = _get("rv32i_tuned")
+    seed = registry.SEED_CFG[3]
+    progs = rv32i.programs(seed, np.arange(200))
+    budgets = rv32i.budgets(seed, 200, median=400, lo=50, hi=2000)
+    ref = oracle.rv32i_run(progs, seed, budgets)
+    with cd.model(64) as m:
+        m.set_mem_images("imem", progs)
+        got = m.submit_queue(seed, budgets)
+    assert (got == ref).all()
