@@ -12,12 +12,15 @@ of lanes). One gate is then one bit-select instruction pair for 64 stimuli.
 hash-consed DAG of single-bit nodes (AND, XOR, MUX over complementable
 literals), following the cycle protocol the per-lane kernels implement:
 
-  phase A  inputs of cycle c, clk = 0: combinational logic -> next state (D)
+  A(c)     inputs of cycle c, clk = 0: combinational logic -> next state (D)
   commit   state <- D for active lanes (the clk = 1 edge: every clocked block
            reads pre-edge values, non-blocking semantics)
-  phase B  clk = 1: combinational logic -> hashed output bits
-  phase F  after the last cycle: every stored variable, for the write-back
+  B(c)     clk = 1: combinational logic -> hashed output bits
+  F        after the last cycle: every stored variable, for the write-back
            to the lane-minor state arrays
+
+B(c) and A(c + 1) both read the state after cycle c's edge, so the kernel
+runs them as one merged pass (program M) over two sets of input planes.
 
 Each phase keeps only the cone of what it produces and is levelised; the
 runtime kernel (csrc/rf_bitplane.hip) evaluates one level per barrier with
@@ -45,6 +48,9 @@ MAX_MUL_WIDTH = 16
 
 class Unsupported(Exception):
     pass
+
+
+LEAVES = ("c0", "pi", "pib", "st")
 
 
 class Net:
@@ -212,6 +218,7 @@ class Lowerer:
         self.d = comp.design
         self.net = Net()
         self.pi: Dict[Var, List[Lit]] = {}
+        self.pi_b: Dict[Var, List[Lit]] = {}
         self.state: Dict[Var, List[Lit]] = {}
 
     # ----------------------------------------------------------- exprs
@@ -425,14 +432,14 @@ class Lowerer:
         return self.state[v]
 
     # -------------------------------------------------------- the design
-    def comb_env(self, clk_val: Lit) -> Dict[Var, List[Lit]]:
+    def comb_env(self, clk_val: Lit, pi: Dict[Var, List[Lit]]) -> Dict[Var, List[Lit]]:
         comp = self.comp
         env: Dict[Var, List[Lit]] = {}
         # variables no process drives and nothing stores read as zero
         for v in self.d.vars:
             if not v.is_mem:
                 env[v] = [F] * v.width
-        env.update(self.pi)
+        env.update(pi)
         env.update(self.state)
         for c in self.d.clocks:
             env[c] = [clk_val]
@@ -471,9 +478,12 @@ class Lowerer:
                 if any(edge != "posedge" or c is not clk for edge, c in p.edges):
                     raise Unsupported("clocked block not on the posedge of the clock")
         # leaves
+        # two sets of input leaves: the clk = 0 settle of cycle n + 1 and the
+        # clk = 1 settle of cycle n run in one pass over different inputs
         for v in d.vars:
             if v.kind == "input" and v not in d.clocks:
                 self.pi[v] = [n.leaf("pi", (v, i)) for i in range(v.width)]
+                self.pi_b[v] = [n.leaf("pib", (v, i)) for i in range(v.width)]
         # state: every stored variable that no combinational block drives
         # (clocked registers, and registers only initial blocks set),
         # loaded from the lane-minor arrays when a launch starts
@@ -487,7 +497,7 @@ class Lowerer:
         for v in state_vars:
             self.state[v] = [n.leaf("st", (v, i)) for i in range(v.width)]
         # phase A: clk = 0 settle, then the clocked blocks read pre-edge values
-        envA = self.comb_env(F)
+        envA = self.comb_env(F, self.pi)
         nxt: Dict[Var, List[Lit]] = {}
         written_by: Dict[Var, int] = {}
         for vx in seq_vx:
@@ -514,7 +524,7 @@ class Lowerer:
             if v not in nxt:
                 nxt[v] = self.state[v]
         # phase B / F: clk = 1 settle on the committed state
-        envB = self.comb_env(T)
+        envB = self.comb_env(T, self.pi_b)
         self.envB = envB
         self.nxt = nxt
         self.state_vars = state_vars
@@ -547,7 +557,7 @@ def _cone(net: Net, roots: Sequence[Lit]) -> List[int]:
         if x in seen:
             continue
         seen.add(x)
-        if net.kind[x] in ("c0", "pi", "st"):
+        if net.kind[x] in LEAVES:
             continue
         stack.append((x, True))
         for o in net.ops[x]:
@@ -644,13 +654,20 @@ CLS_SIZE = {"C": 1, "S": 2, "I": 4, "Q": 8, "W": 4}
 
 
 def compile_program(comp) -> Program:
-    """Lower and schedule; raises Unsupported for designs outside the subset."""
+    """Lower and schedule; raises Unsupported for designs outside the subset.
+
+    Programs: A0 (clk = 0 settle of the first cycle -> next state), M (the
+    clk = 1 settle of cycle n -> hashed outputs, merged with the clk = 0
+    settle of cycle n + 1 -> next state; both read the state after cycle n's
+    edge), B (the clk = 1 settle of the last cycle), and the write-back
+    chunks F."""
     from .layout import class_of
     lw = Lowerer(comp).lower()
     net = lw.net
     d = comp.design
     lay = comp.layout
-    # fixed slots: 0 = zeros, then input bits, then state bits
+    # fixed slots: 0 = zeros, then the inputs of the clk = 0 settle (set A)
+    # and of the clk = 1 settle (set B), then the state bits
     fixed: Dict[int, int] = {0: 0}
     top = 1
     in_chunks = []
@@ -658,13 +675,14 @@ def compile_program(comp) -> Program:
         if port["dir"] != 0:
             continue
         v = port["var"]
-        bits = lw.pi[v]
         for j in range((v.width + 63) // 64):
             nb = min(64, v.width - 64 * j)
-            in_chunks.append(((port["stim_index"] << 3) | j, nb, top))
+            sa, sb = top, top + nb
             for i in range(nb):
-                fixed[bits[64 * j + i] >> 1] = top + i
-            top += nb
+                fixed[lw.pi[v][64 * j + i] >> 1] = sa + i
+                fixed[lw.pi_b[v][64 * j + i] >> 1] = sb + i
+            in_chunks.append(((port["stim_index"] << 3) | j, nb, sa, sb))
+            top += 2 * nb
     state_load = []
     for v in lw.state_vars:
         bits = lw.state[v]
@@ -673,12 +691,6 @@ def compile_program(comp) -> Program:
             fixed[b >> 1] = top + i
         top += v.width
     nfixed = top
-    # phase A: next-state bits. A next-state literal that is a leaf or
-    # inverted gets a copy node so commit reads a slot of its own.
-    commit_lits = []
-    for v in lw.state_vars:
-        for i, lit in enumerate(lw.nxt[v]):
-            commit_lits.append((lw.state[v][i] >> 1, lit))
     copies: Dict[Lit, Lit] = {}
 
     def own(lit: Lit) -> Lit:
@@ -691,30 +703,41 @@ def compile_program(comp) -> Program:
             copies[lit] = 2 * n
         return copies[lit]
 
-    roots = []
-    for st_node, lit in commit_lits:
-        node = lit >> 1
-        roots.append(own(lit) if (lit & 1) or net.kind[node] in ("c0", "pi", "st") else lit)
-    phA, slotA, peakA = build_phase(net, roots, fixed, nfixed)
-    commit = [(fixed[st_node], slotA[r >> 1]) for (st_node, _), r in zip(commit_lits, roots)]
-    # phase B: hashed outputs after the edge. The hash of cycle n runs while
-    # the inputs of cycle n + 1 are written, so an output bit that is an
-    # input bit is read from a copy.
+    # next-state roots: a literal that is a leaf or inverted gets a copy node
+    # so the commit reads a slot of its own
+    commit_lits = []
+    for v in lw.state_vars:
+        for i, lit in enumerate(lw.nxt[v]):
+            commit_lits.append((lw.state[v][i] >> 1, lit))
+    droots = [own(lit) if (lit & 1) or net.kind[lit >> 1] in LEAVES else lit for _, lit in commit_lits]
+    # hashed output roots: the hash of cycle n runs after the next commit and
+    # while the inputs of later cycles are written, so an output bit that is
+    # an input or state bit is copied
     out_ports = [p for p in comp.port_table() if p["hashed"]]
     hbits = {}
     for p in out_ports:
         v = p["var"]
-        hbits[v] = [own(b & ~1) ^ (b & 1) if net.kind[b >> 1] == "pi" else b for b in lw.envB[v]]
+        hbits[v] = [own(b & ~1) ^ (b & 1) if net.kind[b >> 1] in ("pi", "pib", "st") else b
+                    for b in lw.envB[v]]
     hroots = [b for p in out_ports for b in hbits[p["var"]]]
+
+    def commit_of(slot):
+        return [(fixed[st_node], slot[r >> 1]) for (st_node, _), r in zip(commit_lits, droots)]
+
+    def hash_of(slot):
+        out = []
+        for p in out_ports:
+            v = p["var"]
+            nbytes = CLS_SIZE[class_of(v)] * (v.slots if v.width > 64 else 1)
+            out.append((nbytes, v.width, [_lit_ref(b, slot) for b in hbits[v]]))
+        return out
+
+    phA, slotA, peakA = build_phase(net, droots, fixed, nfixed)
+    phM, slotM, peakM = build_phase(net, droots + hroots, fixed, nfixed)
     phB, slotB, peakB = build_phase(net, hroots, fixed, nfixed)
-    hash_ports = []
-    for p in out_ports:
-        v = p["var"]
-        nbytes = CLS_SIZE[class_of(v)] * (v.slots if v.width > 64 else 1)
-        hash_ports.append((nbytes, v.width, [_lit_ref(b, slotB) for b in hbits[v]]))
     # phase F: every stored variable after the last cycle, in chunks whose
     # values all stay live until their write-back; chunks are sized so that
-    # phase F needs no more slots than phases A and B
+    # phase F needs no more slots than the per-cycle phases
     wb = []
     for v in d.vars:
         if v.is_mem or not lay.has(v):
@@ -722,7 +745,7 @@ def compile_program(comp) -> Program:
         bits = _final_bits(comp, lw, v)
         if bits is not None:
             wb.append((v, bits))
-    target = max(peakA, peakB)
+    target = max(peakA, peakM, peakB)
     k = max(1, sum(len(b) for _, b in wb))
     while True:
         groups, cur, n = [], [], 0
@@ -749,15 +772,15 @@ def compile_program(comp) -> Program:
         k //= 2
     state_load.sort(key=lambda e: e[3] != 1)
     prog = Program()
-    prog.nslots = max(peakA, peakB, peakF)
+    prog.nslots = max(target, peakF)
     if prog.nslots > MAX_SLOTS:
         raise Unsupported(f"{prog.nslots} live bit slots exceed LDS")
     prog.in_chunks = in_chunks
     prog.state_load = state_load
-    prog.phases = [phA, phB]
+    prog.phases = [phA, phM, phB]
+    prog.commits = [commit_of(slotA), commit_of(slotM)]
+    prog.hashes = [hash_of(slotM), hash_of(slotB)]
     prog.fchunks = fchunks
-    prog.commit = commit
-    prog.hash_ports = hash_ports
     prog.nodes = len(net.kind)
     prog.gates = [sum(len(l) for l in ph.levels) for ph in prog.phases] + \
         [sum(len(l) for ph, _ in fchunks for l in ph.levels)]
@@ -786,7 +809,7 @@ def _final_bits(comp, lw: Lowerer, v: Var) -> Optional[List[Lit]]:
 # =====================================================================
 
 BP_MAGIC = 0x50424652   # "RFBP"
-BP_VERSION = 4
+BP_VERSION = 5
 HDR_WORDS = 18
 FCHUNK_HDR = 6
 
@@ -807,18 +830,20 @@ def encode(prog: Program) -> List[int]:
     """u32 words of the program blob.
 
     header (18 words): magic, version, nslots, n_in, n_state, n_commit,
-    n_hash, n_fchunks, nlev[A, B], ngates[A, B], n_state1, total words, 0 x 4.
-    The first n_state1 state entries are single-bit. Then, each section
-    starting at an even word offset:
-      inputs      n_in  x {k, nbits, slot0}: bits of stimulus stream k into
-                  slots slot0 .. slot0 + nbits - 1
+    n_hash, n_fchunks, nlev[A0, M, B], ngates[A0, M, B], n_state1, total
+    words, 0, 0. The first n_state1 state entries are single-bit. Then, each
+    section starting at an even word offset:
+      inputs      n_in x {k, nbits, slot_a, slot_b}: bits of stimulus stream k
+                  into slots slot_a ..; slot_b .. is the copy the clk = 1
+                  settle reads
       state load  n_state x {cls, loc_lo, loc_hi, words, width, slot0}
-      commit      n_commit x {state slot, next-state slot}
-      phase A, B  level offsets (nlev + 1 gate indices, padded to even),
+      commit A0   n_commit x {state slot, next-state slot} after phase A0
+      commit M    the same after phase M
+      phase A0, M, B: level offsets (nlev + 1 gate indices, padded to even),
                   then the gates as u64 (lo word first):
                   dst | s << 16 | x << 32 | ix << 47 | y << 48 | iy << 63,
                   dst = s ? (x ^ ix) : (y ^ iy) on 64-lane planes
-      hash        n_hash x {nbytes, width, width slot refs}
+      hash M, hash B: n_hash x {nbytes, width, width slot refs} each
       F chunks    n_fchunks x {nlev, ngates, n_wb, n_wb1, words, 0, level
                   offsets (padded to even), gates, write-back entries
                   {cls, loc_lo, loc_hi, words, width, width slot refs}, the
@@ -844,20 +869,22 @@ def encode(prog: Program) -> List[int]:
         return w
 
     pad(out)
-    for k, nb, s0 in prog.in_chunks:
-        out += [k, nb, s0]
+    for k, nb, sa, sb in prog.in_chunks:
+        out += [k, nb, sa, sb]
     pad(out)
     for cls, loc, words, width, s0 in prog.state_load:
         out += [cls, loc & 0xffffffff, loc >> 32, words, width, s0]
-    pad(out)
-    for st, dn in prog.commit:
-        out += [st, dn]
+    for commit in prog.commits:
+        pad(out)
+        for st, dn in commit:
+            out += [st, dn]
     for ph in prog.phases:
         pad(out)
         out += phase_words(ph)
-    pad(out)
-    for nbytes, width, refs in prog.hash_ports:
-        out += [nbytes, width] + refs
+    for hports in prog.hashes:
+        pad(out)
+        for nbytes, width, refs in hports:
+            out += [nbytes, width] + refs
     pad(out)
     for ph, entries in prog.fchunks:
         body = phase_words(ph) + _wb_words(entries)
@@ -867,10 +894,10 @@ def encode(prog: Program) -> List[int]:
                 FCHUNK_HDR + len(body), 0] + body
     pad(out)
     hdr = [BP_MAGIC, BP_VERSION, prog.nslots, len(prog.in_chunks), len(prog.state_load),
-           len(prog.commit), len(prog.hash_ports), len(prog.fchunks)]
+           len(prog.commits[0]), len(prog.hashes[0]), len(prog.fchunks)]
     hdr += [len(ph.levels) for ph in prog.phases]
     hdr += [sum(len(l) for l in ph.levels) for ph in prog.phases]
-    hdr += [sum(1 for e in prog.state_load if e[3] == 1), len(out), 0, 0, 0, 0]
+    hdr += [sum(1 for e in prog.state_load if e[3] == 1), len(out), 0, 0]
     out[:HDR_WORDS] = hdr
     return out
 
@@ -881,25 +908,27 @@ def sections(blob: Sequence[int]) -> Dict[str, object]:
     h = blob[:HDR_WORDS]
     assert h[0] == BP_MAGIC and h[1] == BP_VERSION
     n_in, n_state, n_commit, n_hash, n_f = h[3:8]
-    nlev = h[8:10]
-    ngates = h[10:12]
+    nlev = h[8:11]
+    ngates = h[11:14]
     o = HDR_WORDS
     ev = lambda x: x + (x & 1)
     res: Dict[str, object] = {}
     res["in"] = o = ev(o)
-    o += 3 * n_in
+    o += 4 * n_in
     res["state"] = o = ev(o)
     o += 6 * n_state
-    res["commit"] = o = ev(o)
-    o += 2 * n_commit
-    for k, name in enumerate("AB"):
+    for name in ("commitA0", "commitM"):
+        res[name] = o = ev(o)
+        o += 2 * n_commit
+    for k, name in enumerate(("A0", "M", "B")):
         res["lev" + name] = o = ev(o)
         o += nlev[k] + 1
         res["gates" + name] = o = ev(o)
         o += 2 * ngates[k]
-    res["hash"] = o = ev(o)
-    for _ in range(n_hash):
-        o += 2 + blob[o + 1]
+    for name in ("hashM", "hashB"):
+        res[name] = o = ev(o)
+        for _ in range(n_hash):
+            o += 2 + blob[o + 1]
     o = ev(o)
     chunks = []
     for _ in range(n_f):
@@ -911,5 +940,5 @@ def sections(blob: Sequence[int]) -> Dict[str, object]:
         o += words
     res["fchunks"] = chunks
     res["end"] = ev(o)
-    assert res["end"] == h[13], (res["end"], h[13])
+    assert res["end"] == h[15], (res["end"], h[15])
     return res

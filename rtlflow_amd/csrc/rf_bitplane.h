@@ -12,27 +12,31 @@
 
 #define RF_BP_DEFAULT_THREADS 512
 #define RF_BP_MAGIC 0x50424652u
-#define RF_BP_VERSION 4u
+#define RF_BP_VERSION 5u
 #define RF_BP_FCHUNK_HDR 6u
 #define RF_BP_HDR_WORDS 18u
 
-#define RF_BP_MAXR 48   // most gate rounds of phases A + B held in registers
+#define RF_BP_MAXR 48   // most gate rounds of the merged per-cycle pass held in registers
+
+// Programs of a blob: A0 (first cycle's next state), M (outputs of cycle n
+// merged with the next state of cycle n + 1), B (last cycle's outputs).
+enum { RF_BP_A0 = 0, RF_BP_M = 1, RF_BP_B = 2 };
 
 struct rf_bp_params {
     rf_args a;
-    // phases A and B as rounds of one gate per thread, for the kernels that
-    // keep every thread's gate records in registers across cycles: round i
-    // covers the gate records [rb[i], re[i]) (uint2 index from ``prog``),
-    // thread t takes record rb[i] + t; bit i of ``bar``: a barrier follows.
-    uint32_t nr_a, nr_b;
+    // program M as rounds of one gate per thread, for the kernels that keep
+    // every thread's gate records in registers across cycles: round i covers
+    // the gate records [rb[i], re[i]) (uint2 index from ``prog``), thread t
+    // takes record rb[i] + t; bit i of ``bar``: a barrier follows.
+    uint32_t nr;
     uint64_t bar;
     uint32_t rb[RF_BP_MAXR], re[RF_BP_MAXR];
     const uint32_t* prog;     // device copy of the blob
     uint32_t nslots;
     uint32_t n_in, n_state, n_commit, n_hash, n_f;
     uint32_t n_state1;          // leading single-bit entries of the state list
-    uint32_t off_in, off_state, off_commit, off_hash, off_f;
-    uint32_t nlev[2], off_lev[2], off_gates[2];   // phases A and B
+    uint32_t off_in, off_state, off_commit[2], off_hash[2], off_f;
+    uint32_t nlev[3], off_lev[3], off_gates[3];
 };
 
 // Parse and check a blob (every slot index, level table and section size);
@@ -40,8 +44,8 @@ struct rf_bp_params {
 // with a reason in *why.
 int rf_bp_parse(const uint32_t* blob, uint64_t words, rf_bp_params* p, const char** why);
 
-// Fill the round table of phases A and B for ``threads`` threads per
-// workgroup; nr_a = nr_b = 0 when they need more than RF_BP_MAXR rounds.
+// Fill the round table of program M for ``threads`` threads per
+// workgroup; nr = 0 when it needs more than RF_BP_MAXR rounds.
 void rf_bp_rounds(const uint32_t* blob, uint32_t threads, rf_bp_params* p);
 
 // Launch one kernel over ceil(nlanes / 64) workgroups of ``threads``

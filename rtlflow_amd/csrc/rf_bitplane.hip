@@ -124,20 +124,30 @@ __device__ __forceinline__ void bp_store(const rf_args& a, uint32_t cls, uint64_
     }
 }
 
-// Inputs of one cycle into their planes (wavefront 0).
+// Inputs of one cycle into their planes (wavefront 0): set A, read by the
+// clk = 0 settle.
 __device__ __forceinline__ void bp_inputs(uint64_t* L, const rf_bp_params& P, uint64_t sid, uint32_t cyc,
                                           uint32_t t) {
     const uint32_t* ic = P.prog + P.off_in;
-    for (uint32_t k = 0; k < P.n_in; ++k, ic += 3) {
+    for (uint32_t k = 0; k < P.n_in; ++k, ic += 4) {
         const uint64_t r = rf_stim(P.a.seed, sid, (uint64_t)cyc, ic[0]);
         bp_to_planes(L, ic[2], r, ic[1], t);
     }
 }
 
-// Fold this cycle's outputs into the lane's hash (wavefront 1).
-__device__ __forceinline__ uint64_t bp_hash(const uint64_t* L, const rf_bp_params& P, uint64_t h,
-                                            uint32_t t) {
-    const uint32_t* hp = P.prog + P.off_hash;
+// Input set A -> set B (the clk = 1 settle of the same cycle reads set B
+// while set A already holds the next cycle's inputs); wavefront 0.
+__device__ __forceinline__ void bp_copy_inputs(uint64_t* L, const rf_bp_params& P, uint32_t t) {
+    const uint32_t* ic = P.prog + P.off_in;
+    for (uint32_t k = 0; k < P.n_in; ++k, ic += 4)
+        if (t < ic[1]) L[ic[3] + t] = L[ic[2] + t];
+}
+
+// Fold this cycle's outputs into the lane's hash (wavefront 1); ``which``
+// selects the output-slot table of program M or B.
+__device__ __forceinline__ uint64_t bp_hash(const uint64_t* L, const rf_bp_params& P, uint32_t which,
+                                            uint64_t h, uint32_t t) {
+    const uint32_t* hp = P.prog + P.off_hash[which];
     for (uint32_t k = 0; k < P.n_hash; ++k) {
         const uint32_t nbytes = hp[0], width = hp[1];
         const uint32_t* refs = hp + 2;
@@ -152,6 +162,16 @@ __device__ __forceinline__ uint64_t bp_hash(const uint64_t* L, const rf_bp_param
         hp += 2 + width;
     }
     return h;
+}
+
+// Commit: state <- next state for active lanes (the clock edge).
+template <int NT>
+__device__ __forceinline__ void bp_commit(uint64_t* L, const uint32_t* c, uint32_t n, uint64_t am,
+                                          uint32_t tid) {
+    for (uint32_t i = tid; i < n; i += NT) {
+        const uint32_t st = c[2 * i], dn = c[2 * i + 1];
+        L[st] = (L[dn] & am) | (L[st] & ~am);
+    }
 }
 
 // Rounds [i0, i1) of the register-resident gate records, a barrier where
@@ -252,13 +272,12 @@ __global__ void __launch_bounds__(NT) rf_bp_kernel(rf_bp_params P) {
     }
     if (w0 && a.ncycles) bp_inputs(L, P, sid, cyc, t);
     __syncthreads();
-    const uint32_t* lev[2];
-    const uint2* gates[2];
-    for (int k = 0; k < 2; ++k) {
+    const uint32_t* lev[3];
+    const uint2* gates[3];
+    for (int k = 0; k < 3; ++k) {
         lev[k] = prog + P.off_lev[k];
         gates[k] = reinterpret_cast<const uint2*>(prog + P.off_gates[k]);
     }
-    const uint32_t* commit = prog + P.off_commit;
     const uint64_t am = s_act;
     constexpr int MR = MAXR > 0 ? MAXR : 1;
     uint2 d[MR];
@@ -267,31 +286,45 @@ __global__ void __launch_bounds__(NT) rf_bp_kernel(rf_bp_params P) {
 #pragma unroll
         for (int i = 0; i < MAXR; ++i) {
             d[i] = make_uint2(0xffffffffu, 0u);
-            if (i < (int)(P.nr_a + P.nr_b)) {
+            if (i < (int)P.nr) {
                 const uint32_t idx = P.rb[i] + tid;
                 if (idx < P.re[i]) d[i] = pg[idx];
             }
         }
     }
-    for (uint32_t n = 0; n < a.ncycles; ++n) {
-        if constexpr (MAXR > 0)
-            bp_rounds<MR>(L, d, P, 0, (int)P.nr_a);
-        else
-            bp_phase<NT>(L, lev[0], gates[0], P.nlev[0], tid);
-        for (uint32_t i = tid; i < P.n_commit; i += NT) {
-            const uint32_t st = commit[2 * i], dn = commit[2 * i + 1];
-            L[st] = (L[dn] & am) | (L[st] & ~am);
+    const uint32_t N = a.ncycles;
+    if (N) {
+        // cycle 0's next state
+        bp_phase<NT>(L, lev[RF_BP_A0], gates[RF_BP_A0], P.nlev[RF_BP_A0], tid);
+        bp_commit<NT>(L, prog + P.off_commit[0], P.n_commit, am, tid);
+        __syncthreads();
+        if (w0) {
+            bp_copy_inputs(L, P, t);
+            if (N > 1) bp_inputs(L, P, sid, cyc + 1, t);
         }
         __syncthreads();
-        if constexpr (MAXR > 0)
-            bp_rounds<MR>(L, d, P, (int)P.nr_a, (int)(P.nr_a + P.nr_b));
-        else
-            bp_phase<NT>(L, lev[1], gates[1], P.nlev[1], tid);
+        // cycle n's outputs and cycle n + 1's next state in one pass
+        for (uint32_t n = 0; n + 1 < N; ++n) {
+            if constexpr (MAXR > 0)
+                bp_rounds<MR>(L, d, P, 0, (int)P.nr);
+            else
+                bp_phase<NT>(L, lev[RF_BP_M], gates[RF_BP_M], P.nlev[RF_BP_M], tid);
+            bp_commit<NT>(L, prog + P.off_commit[1], P.n_commit, am, tid);
+            __syncthreads();
+            if (w1) {
+                const uint64_t hn = bp_hash(L, P, 0, h, t);
+                if (act) h = hn;
+            } else if (w0) {
+                bp_copy_inputs(L, P, t);
+                if (n + 2 < N) bp_inputs(L, P, sid, cyc + n + 2, t);
+            }
+            __syncthreads();
+        }
+        // the last cycle's outputs
+        bp_phase<NT>(L, lev[RF_BP_B], gates[RF_BP_B], P.nlev[RF_BP_B], tid);
         if (w1) {
-            const uint64_t hn = bp_hash(L, P, h, t);
+            const uint64_t hn = bp_hash(L, P, 1, h, t);
             if (act) h = hn;
-        } else if (w0 && n + 1 < a.ncycles) {
-            bp_inputs(L, P, sid, cyc + n + 1, t);
         }
         __syncthreads();
     }
@@ -379,25 +412,29 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
     p->n_commit = b[5];
     p->n_hash = b[6];
     p->n_f = b[7];
-    p->nlev[0] = b[8];
-    p->nlev[1] = b[9];
-    const uint32_t ngates[2] = {b[10], b[11]};
-    p->n_state1 = b[12];
-    if (b[13] != words) BP_FAIL("bit-plane program size mismatch");
+    uint32_t ngates[3];
+    for (int k = 0; k < 3; ++k) {
+        p->nlev[k] = b[8 + k];
+        ngates[k] = b[11 + k];
+    }
+    p->n_state1 = b[14];
+    if (b[15] != words) BP_FAIL("bit-plane program size mismatch");
     if (p->n_state1 > p->n_state) BP_FAIL("bit-plane single-bit count exceeds the state list");
     if (p->nslots == 0 || p->nslots > 0x7fffu || (uint64_t)p->nslots * 8u > 160u * 1024u - 64u)
         BP_FAIL("bit-plane program needs more LDS than a workgroup has");
     auto ev = [](uint64_t x) { return x + (x & 1); };
     uint64_t o = ev(RF_BP_HDR_WORDS);
     p->off_in = (uint32_t)o;
-    o += 3ull * p->n_in;
+    o += 4ull * p->n_in;
     o = ev(o);
     p->off_state = (uint32_t)o;
     o += 6ull * p->n_state;
-    o = ev(o);
-    p->off_commit = (uint32_t)o;
-    o += 2ull * p->n_commit;
     for (int k = 0; k < 2; ++k) {
+        o = ev(o);
+        p->off_commit[k] = (uint32_t)o;
+        o += 2ull * p->n_commit;
+    }
+    for (int k = 0; k < 3; ++k) {
         o = ev(o);
         p->off_lev[k] = (uint32_t)o;
         o += p->nlev[k] + 1ull;
@@ -406,20 +443,22 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
         o += 2ull * ngates[k];
     }
     if (o > words) BP_FAIL("bit-plane sections exceed the program");
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < 3; ++k)
         if (!bp_check_phase(b + p->off_lev[k], p->nlev[k], b + p->off_gates[k], ngates[k], p->nslots))
-            BP_FAIL("bit-plane phase A/B malformed");
-    o = ev(o);
-    p->off_hash = (uint32_t)o;
-    for (uint32_t k = 0; k < p->n_hash; ++k) {
-        if (o + 2 > words) BP_FAIL("bit-plane hash section truncated");
-        const uint32_t nbytes = b[o], width = b[o + 1];
-        if (width == 0 || o + 2 + width > words) BP_FAIL("bit-plane hash entry malformed");
-        if ((width <= 64 && (nbytes > 8 || nbytes * 8u < width)) ||
-            (width > 64 && (nbytes % 4u || nbytes * 8u < width || nbytes * 8u >= width + 32u)))
-            BP_FAIL("bit-plane hash entry malformed");
-        if (!bp_check_refs(b + o + 2, width, p->nslots)) BP_FAIL("bit-plane hash slot out of range");
-        o += 2ull + width;
+            BP_FAIL("bit-plane program A0/M/B malformed");
+    for (int which = 0; which < 2; ++which) {
+        o = ev(o);
+        p->off_hash[which] = (uint32_t)o;
+        for (uint32_t k = 0; k < p->n_hash; ++k) {
+            if (o + 2 > words) BP_FAIL("bit-plane hash section truncated");
+            const uint32_t nbytes = b[o], width = b[o + 1];
+            if (width == 0 || o + 2 + width > words) BP_FAIL("bit-plane hash entry malformed");
+            if ((width <= 64 && (nbytes > 8 || nbytes * 8u < width)) ||
+                (width > 64 && (nbytes % 4u || nbytes * 8u < width || nbytes * 8u >= width + 32u)))
+                BP_FAIL("bit-plane hash entry malformed");
+            if (!bp_check_refs(b + o + 2, width, p->nslots)) BP_FAIL("bit-plane hash slot out of range");
+            o += 2ull + width;
+        }
     }
     o = ev(o);
     p->off_f = (uint32_t)o;
@@ -447,8 +486,9 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
     }
     if (ev(o) != words) BP_FAIL("bit-plane program sections do not add up");
     for (uint32_t k = 0; k < p->n_in; ++k) {
-        const uint32_t* ic = b + p->off_in + 3 * k;
-        if (ic[1] > 64 || ic[2] + ic[1] > p->nslots) BP_FAIL("bit-plane input slot out of range");
+        const uint32_t* ic = b + p->off_in + 4 * k;
+        if (ic[1] == 0 || ic[1] > 64 || ic[2] + ic[1] > p->nslots || ic[3] + ic[1] > p->nslots)
+            BP_FAIL("bit-plane input slot out of range");
     }
     for (uint32_t k = 0; k < p->n_state; ++k) {
         const uint32_t* sl = b + p->off_state + 6 * k;
@@ -457,36 +497,35 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
             (sl[4] > 64 && (sl[0] != RF_CLS_I || sl[3] * 32u < sl[4] || sl[3] * 32u >= sl[4] + 32u)))
             BP_FAIL("bit-plane state entry out of range");
     }
-    for (uint32_t k = 0; k < p->n_commit; ++k) {
-        const uint32_t* c = b + p->off_commit + 2 * k;
-        if (c[0] >= p->nslots || c[1] >= p->nslots) BP_FAIL("bit-plane commit slot out of range");
-    }
+    for (int which = 0; which < 2; ++which)
+        for (uint32_t k = 0; k < p->n_commit; ++k) {
+            const uint32_t* c = b + p->off_commit[which] + 2 * k;
+            if (c[0] >= p->nslots || c[1] >= p->nslots) BP_FAIL("bit-plane commit slot out of range");
+        }
     return 0;
 #undef BP_FAIL
 }
 
 void rf_bp_rounds(const uint32_t* b, uint32_t threads, rf_bp_params* p) {
-    p->nr_a = p->nr_b = 0;
+    p->nr = 0;
     p->bar = 0;
     uint32_t r = 0;
-    uint32_t nr[2] = {0, 0};
-    for (int k = 0; k < 2; ++k) {
-        const uint32_t* lv = b + p->off_lev[k];
-        const uint32_t g = p->off_gates[k] / 2;   // uint2 index of the phase's first record
-        for (uint32_t l = 0; l < p->nlev[k]; ++l) {
-            const uint32_t lo = lv[l], hi = lv[l + 1];
-            for (uint32_t s = lo; s < hi; s += threads) {
-                if (r >= RF_BP_MAXR) return;
-                p->rb[r] = g + s;
-                p->re[r] = g + hi;
-                if (s + threads >= hi) p->bar |= 1ull << r;
-                ++r;
-                ++nr[k];
+    const uint32_t* lv = b + p->off_lev[RF_BP_M];
+    const uint32_t g = p->off_gates[RF_BP_M] / 2;   // uint2 index of the first record
+    for (uint32_t l = 0; l < p->nlev[RF_BP_M]; ++l) {
+        const uint32_t lo = lv[l], hi = lv[l + 1];
+        for (uint32_t s = lo; s < hi; s += threads) {
+            if (r >= RF_BP_MAXR) {
+                p->bar = 0;
+                return;
             }
+            p->rb[r] = g + s;
+            p->re[r] = g + hi;
+            if (s + threads >= hi) p->bar |= 1ull << r;
+            ++r;
         }
     }
-    p->nr_a = nr[0];
-    p->nr_b = nr[1];
+    p->nr = r;
 }
 
 template <int NT, int MAXR>
@@ -506,8 +545,8 @@ static hipError_t bp_launch_k(const rf_bp_params& p, hipStream_t stream) {
 
 template <int NT>
 static hipError_t bp_launch_nt(const rf_bp_params& p, hipStream_t stream) {
-    const uint32_t r = p.nr_a + p.nr_b;
-    if (p.nr_a == 0 || r > RF_BP_MAXR) return bp_launch_k<NT, 0>(p, stream);
+    const uint32_t r = p.nr;
+    if (r == 0 || r > RF_BP_MAXR) return bp_launch_k<NT, 0>(p, stream);
     if (r <= 16) return bp_launch_k<NT, 16>(p, stream);
     if (r <= 24) return bp_launch_k<NT, 24>(p, stream);
     if (r <= 32) return bp_launch_k<NT, 32>(p, stream);

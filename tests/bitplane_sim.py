@@ -53,21 +53,25 @@ class Program:
         self.sec = bp.sections(blob)
         h = blob[:bp.HDR_WORDS]
         self.nslots, self.n_in, self.n_state, self.n_commit, self.n_hash = h[2:7]
-        self.nlev = h[8:10]
-        self.ngates = h[10:12]
+        self.nlev = h[8:11]
+        self.ngates = h[11:14]
         b = self.b
         s = self.sec
-        self.inputs = b[s["in"]:s["in"] + 3 * self.n_in].reshape(-1, 3).astype(np.int64)
+        self.inputs = b[s["in"]:s["in"] + 4 * self.n_in].reshape(-1, 4).astype(np.int64)
         self.state = b[s["state"]:s["state"] + 6 * self.n_state].reshape(-1, 6).astype(np.int64)
-        self.commit = b[s["commit"]:s["commit"] + 2 * self.n_commit].reshape(-1, 2).astype(np.int64)
+        self.commits = [b[s[nm]:s[nm] + 2 * self.n_commit].reshape(-1, 2).astype(np.int64)
+                        for nm in ("commitA0", "commitM")]
         self.phases = [self._phase(s["lev" + nm], self.nlev[k], s["gates" + nm], self.ngates[k])
-                       for k, nm in enumerate("AB")]
-        o = s["hash"]
-        self.hash = []
-        for _ in range(self.n_hash):
-            nbytes, width = int(b[o]), int(b[o + 1])
-            self.hash.append((nbytes, width, b[o + 2:o + 2 + width].astype(np.int64)))
-            o += 2 + width
+                       for k, nm in enumerate(("A0", "M", "B"))]
+        self.hashes = []
+        for nm in ("hashM", "hashB"):
+            o = s[nm]
+            hp = []
+            for _ in range(self.n_hash):
+                nbytes, width = int(b[o]), int(b[o + 1])
+                hp.append((nbytes, width, b[o + 2:o + 2 + width].astype(np.int64)))
+                o += 2 + width
+            self.hashes.append(hp)
         self.fchunks = []
         for c in s["fchunks"]:
             ph = self._phase(c["lev"], c["nlev"], c["gates"], c["ngates"])
@@ -100,31 +104,60 @@ class Program:
         v = L[r & 0x7FFFFFFF]
         return v ^ M64 if r >> 31 else v
 
-    def run(self, seed, nlanes, ncycles, stim0=0, state=None):
-        """Lanes start from zero state (or ``state``: per-var values) and run
-        ``ncycles`` cycles; returns hashes[nlanes]."""
+    def inputs_to(self, L, seed, sids, c):
+        for k, nb, sa, sb in self.inputs:
+            L[sa:sa + nb] = to_planes(stim(seed, sids, c, int(k)), int(nb))
+
+    def copy_inputs(self, L):
+        for k, nb, sa, sb in self.inputs:
+            L[sb:sb + nb] = L[sa:sa + nb]
+
+    def commit(self, L, k, act):
+        st, dn = self.commits[k][:, 0], self.commits[k][:, 1]
+        L[st] = (L[dn] & act) | (L[st] & ~act)
+
+    def hash(self, L, k, h):
+        for nbytes, width, refs in self.hashes[k]:
+            planes = np.array([self.ref(L, int(r)) for r in refs], np.uint64)
+            if width <= 64:
+                h = fnv_bytes(h, from_planes(planes), nbytes)
+            else:
+                for w0 in range(0, nbytes * 8, 32):
+                    h = fnv_bytes(h, from_planes(planes[w0:w0 + 32]), 4)
+        return h
+
+    def cycles(self, L, seed, sids, ncycles, act, h):
+        """The kernel's schedule: A0, commit; then per cycle the merged pass
+        M (outputs of cycle n, next state of cycle n + 1), commit, hash of
+        cycle n; B and the hash of the last cycle."""
+        if ncycles == 0:
+            return h
+        self.inputs_to(L, seed, sids, 0)
+        self.run_phase(L, 0)
+        self.commit(L, 0, act)
+        self.copy_inputs(L)
+        if ncycles > 1:
+            self.inputs_to(L, seed, sids, 1)
+        for n in range(ncycles - 1):
+            self.run_phase(L, 1)
+            self.commit(L, 1, act)
+            h = self.hash(L, 0, h)
+            self.copy_inputs(L)
+            if n + 2 < ncycles:
+                self.inputs_to(L, seed, sids, n + 2)
+        self.run_phase(L, 2)
+        return self.hash(L, 1, h)
+
+    def run(self, seed, nlanes, ncycles, stim0=0):
+        """Lanes start from zero state and run ``ncycles`` cycles; returns
+        hashes[nlanes]."""
         out = np.zeros(nlanes, np.uint64)
         for g0 in range(0, nlanes, 64):
             n = min(64, nlanes - g0)
             act = np.uint64((1 << n) - 1) if n < 64 else M64
             sids = np.arange(g0 + stim0, g0 + stim0 + 64, dtype=np.uint64)
             L = np.zeros(self.nslots, np.uint64)
-            h = np.full(64, 0xCBF29CE484222325, np.uint64)
-            for c in range(ncycles):
-                for k, nb, s0 in self.inputs:
-                    r = stim(seed, sids, c, int(k))
-                    L[s0:s0 + nb] = to_planes(r, int(nb))
-                self.run_phase(L, 0)
-                st, dn = self.commit[:, 0], self.commit[:, 1]
-                L[st] = (L[dn] & act) | (L[st] & ~act)
-                self.run_phase(L, 1)
-                for nbytes, width, refs in self.hash:
-                    planes = np.array([self.ref(L, int(r)) for r in refs], np.uint64)
-                    if width <= 64:
-                        h = fnv_bytes(h, from_planes(planes), nbytes)
-                    else:
-                        for w0 in range(0, nbytes * 8, 32):
-                            h = fnv_bytes(h, from_planes(planes[w0:w0 + 32]), 4)
+            h = self.cycles(L, seed, sids, ncycles, act, np.full(64, 0xCBF29CE484222325, np.uint64))
             out[g0:g0 + n] = h[:n]
         return out
 
@@ -137,13 +170,7 @@ def run_state(prog: "Program", seed, nlanes, ncycles, lane):
     act = np.uint64((1 << n) - 1) if n < 64 else M64
     sids = np.arange(g0, g0 + 64, dtype=np.uint64)
     L = np.zeros(prog.nslots, np.uint64)
-    for c in range(ncycles):
-        for k, nb, s0 in prog.inputs:
-            L[s0:s0 + nb] = to_planes(stim(seed, sids, c, int(k)), int(nb))
-        prog.run_phase(L, 0)
-        st, dn = prog.commit[:, 0], prog.commit[:, 1]
-        L[st] = (L[dn] & act) | (L[st] & ~act)
-        prog.run_phase(L, 1)
+    prog.cycles(L, seed, sids, ncycles, act, np.full(64, 0xCBF29CE484222325, np.uint64))
     t = lane - g0
     out = {}
     for ph, wb in prog.fchunks:
