@@ -295,9 +295,10 @@ __global__ void __launch_bounds__(NT) rf_bp_kernel(rf_bp_params P) {
     const uint32_t N = a.ncycles;
     if (N) {
         // cycle 0's next state
+        // (the commit writes state slots only; the lane work touches input
+        // and output-copy slots only: one barrier covers both)
         bp_phase<NT>(L, lev[RF_BP_A0], gates[RF_BP_A0], P.nlev[RF_BP_A0], tid);
         bp_commit<NT>(L, prog + P.off_commit[0], P.n_commit, am, tid);
-        __syncthreads();
         if (w0) {
             bp_copy_inputs(L, P, t);
             if (N > 1) bp_inputs(L, P, sid, cyc + 1, t);
@@ -310,7 +311,6 @@ __global__ void __launch_bounds__(NT) rf_bp_kernel(rf_bp_params P) {
             else
                 bp_phase<NT>(L, lev[RF_BP_M], gates[RF_BP_M], P.nlev[RF_BP_M], tid);
             bp_commit<NT>(L, prog + P.off_commit[1], P.n_commit, am, tid);
-            __syncthreads();
             if (w1) {
                 const uint64_t hn = bp_hash(L, P, 0, h, t);
                 if (act) h = hn;
