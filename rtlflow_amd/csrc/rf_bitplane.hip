@@ -200,8 +200,15 @@ __device__ __forceinline__ void bp_rounds(uint64_t* L, const uint2 (&d)[MAXR], c
 // MAXR > 0: every thread keeps its gate records of phases A and B (at most
 // MAXR rounds) in registers for the whole launch, so the cycle loop issues
 // no global-memory loads; MAXR = 0: records are streamed from memory.
+// Up to 24 register-resident rounds the kernel fits in 80 registers, which
+// leaves room for 6 wavefronts per SIMD (3 workgroups of 512 threads per CU
+// when the LDS allows).
+template <int MAXR>
+constexpr int bp_waves_per_eu() { return MAXR > 0 && MAXR <= 24 ? 6 : 1; }
+
 template <int NT, int MAXR>
-__global__ void __launch_bounds__(NT) rf_bp_kernel(rf_bp_params P) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu<MAXR>())))
+rf_bp_kernel(rf_bp_params P) {
     extern __shared__ uint64_t L[];
     __shared__ uint64_t s_act;
     const rf_args& a = P.a;
