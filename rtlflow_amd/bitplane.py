@@ -567,15 +567,32 @@ def _cone(net: Net, roots: Sequence[Lit]) -> List[int]:
 
 
 def build_phase(net: Net, roots: Sequence[Lit], fixed: Dict[int, int], nfixed: int,
-                keep: Sequence[Lit] = ()) -> Tuple[Phase, Dict[int, int], int]:
+                keep: Sequence[Lit] = (), inplace: Sequence[Tuple[int, int]] = ()
+                ) -> Tuple[Phase, Dict[int, int], int]:
     """Levelise the cone of ``roots`` and give each node an LDS slot.
     ``fixed``: node -> slot for leaves. Slots of ``keep`` nodes stay live to
-    the end of the phase. Returns (phase, node->slot, peak slots)."""
+    the end of the phase. ``inplace``: (next-state node, state leaf) pairs; a
+    next-state node whose state leaf has no reader at or after its level
+    writes the state slot directly (the pair is added to ``ph.inplace``).
+    Returns (phase, node->slot, peak slots)."""
     order = _cone(net, roots)
     level: Dict[int, int] = {}
     for x in order:
         level[x] = 1 + max((level.get(o >> 1, 0) for o in net.ops[x]), default=0)
     nlev = max(level.values(), default=0)
+    direct: Dict[int, int] = {}
+    done_pairs = set()
+    if inplace:
+        last_read: Dict[int, int] = {}
+        for x in order:
+            for o in net.ops[x]:
+                y = o >> 1
+                if y not in level:
+                    last_read[y] = max(last_read.get(y, 0), level[x])
+        for d, q in inplace:
+            if d in level and d not in direct and last_read.get(q, 0) < level[d]:
+                direct[d] = fixed[q]
+                done_pairs.add((d, q))
     last: Dict[int, int] = {}
     for x in order:
         for o in net.ops[x]:
@@ -597,10 +614,13 @@ def build_phase(net: Net, roots: Sequence[Lit], fixed: Dict[int, int], nfixed: i
         release.setdefault(l, []).append(x)
     peak = nfixed
     ph = Phase()
+    ph.inplace = done_pairs
     for L in range(1, nlev + 1):
         gates = []
         for x in by_level[L]:
-            if free:
+            if x in direct:
+                s = direct[x]
+            elif free:
                 s = free.pop()
             else:
                 s = top
@@ -612,7 +632,7 @@ def build_phase(net: Net, roots: Sequence[Lit], fixed: Dict[int, int], nfixed: i
         gates.sort(key=lambda g: (g[1], g[2]))
         ph.levels.append(gates)
         for y in release.get(L, []):
-            if y in slot and y not in fixed:
+            if y in slot and y not in fixed and y not in direct:
                 free.append(slot[y])
     return ph, slot, peak
 
@@ -721,8 +741,11 @@ def compile_program(comp) -> Program:
                     for b in lw.envB[v]]
     hroots = [b for p in out_ports for b in hbits[p["var"]]]
 
-    def commit_of(slot):
-        return [(fixed[st_node], slot[r >> 1]) for (st_node, _), r in zip(commit_lits, droots)]
+    def commit_of(ph, slot):
+        return [(fixed[st_node], slot[r >> 1]) for (st_node, _), r in zip(commit_lits, droots)
+                if (r >> 1, st_node) not in ph.inplace]
+
+    pairs = [(r >> 1, st_node) for (st_node, _), r in zip(commit_lits, droots)]
 
     def hash_of(slot):
         out = []
@@ -732,8 +755,8 @@ def compile_program(comp) -> Program:
             out.append((nbytes, v.width, [_lit_ref(b, slot) for b in hbits[v]]))
         return out
 
-    phA, slotA, peakA = build_phase(net, droots, fixed, nfixed)
-    phM, slotM, peakM = build_phase(net, droots + hroots, fixed, nfixed)
+    phA, slotA, peakA = build_phase(net, droots, fixed, nfixed, inplace=pairs)
+    phM, slotM, peakM = build_phase(net, droots + hroots, fixed, nfixed, inplace=pairs)
     phB, slotB, peakB = build_phase(net, hroots, fixed, nfixed)
     # phase F: every stored variable after the last cycle, in chunks whose
     # values all stay live until their write-back; chunks are sized so that
@@ -778,7 +801,7 @@ def compile_program(comp) -> Program:
     prog.in_chunks = in_chunks
     prog.state_load = state_load
     prog.phases = [phA, phM, phB]
-    prog.commits = [commit_of(slotA), commit_of(slotM)]
+    prog.commits = [commit_of(phA, slotA), commit_of(phM, slotM)]
     prog.hashes = [hash_of(slotM), hash_of(slotB)]
     prog.fchunks = fchunks
     prog.nodes = len(net.kind)
@@ -809,7 +832,7 @@ def _final_bits(comp, lw: Lowerer, v: Var) -> Optional[List[Lit]]:
 # =====================================================================
 
 BP_MAGIC = 0x50424652   # "RFBP"
-BP_VERSION = 5
+BP_VERSION = 6
 HDR_WORDS = 18
 FCHUNK_HDR = 6
 
@@ -829,16 +852,18 @@ def _wb_words(entries) -> List[int]:
 def encode(prog: Program) -> List[int]:
     """u32 words of the program blob.
 
-    header (18 words): magic, version, nslots, n_in, n_state, n_commit,
+    header (18 words): magic, version, nslots, n_in, n_state, n_commit[A0],
     n_hash, n_fchunks, nlev[A0, M, B], ngates[A0, M, B], n_state1, total
-    words, 0, 0. The first n_state1 state entries are single-bit. Then, each
-    section starting at an even word offset:
+    words, n_commit[M], 0. The first n_state1 state entries are single-bit.
+    Then, each section starting at an even word offset:
       inputs      n_in x {k, nbits, slot_a, slot_b}: bits of stimulus stream k
                   into slots slot_a ..; slot_b .. is the copy the clk = 1
                   settle reads
       state load  n_state x {cls, loc_lo, loc_hi, words, width, slot0}
-      commit A0   n_commit x {state slot, next-state slot} after phase A0
-      commit M    the same after phase M
+      commit A0   n_commit[A0] x {state slot, next-state slot}: copies after
+                  program A0 (next-state values that the program could not
+                  write into their state slots directly)
+      commit M    n_commit[M] x the same after program M
       phase A0, M, B: level offsets (nlev + 1 gate indices, padded to even),
                   then the gates as u64 (lo word first):
                   dst | s << 16 | x << 32 | ix << 47 | y << 48 | iy << 63,
@@ -897,7 +922,7 @@ def encode(prog: Program) -> List[int]:
            len(prog.commits[0]), len(prog.hashes[0]), len(prog.fchunks)]
     hdr += [len(ph.levels) for ph in prog.phases]
     hdr += [sum(len(l) for l in ph.levels) for ph in prog.phases]
-    hdr += [sum(1 for e in prog.state_load if e[3] == 1), len(out), 0, 0]
+    hdr += [sum(1 for e in prog.state_load if e[3] == 1), len(out), len(prog.commits[1]), 0]
     out[:HDR_WORDS] = hdr
     return out
 
@@ -907,7 +932,8 @@ def sections(blob: Sequence[int]) -> Dict[str, object]:
     computes the same offsets from the header)."""
     h = blob[:HDR_WORDS]
     assert h[0] == BP_MAGIC and h[1] == BP_VERSION
-    n_in, n_state, n_commit, n_hash, n_f = h[3:8]
+    n_in, n_state, n_commit_a, n_hash, n_f = h[3:8]
+    n_commit = (n_commit_a, h[16])
     nlev = h[8:11]
     ngates = h[11:14]
     o = HDR_WORDS
@@ -917,9 +943,9 @@ def sections(blob: Sequence[int]) -> Dict[str, object]:
     o += 4 * n_in
     res["state"] = o = ev(o)
     o += 6 * n_state
-    for name in ("commitA0", "commitM"):
+    for k, name in enumerate(("commitA0", "commitM")):
         res[name] = o = ev(o)
-        o += 2 * n_commit
+        o += 2 * n_commit[k]
     for k, name in enumerate(("A0", "M", "B")):
         res["lev" + name] = o = ev(o)
         o += nlev[k] + 1

@@ -12,7 +12,7 @@
 
 #define RF_BP_DEFAULT_THREADS 512
 #define RF_BP_MAGIC 0x50424652u
-#define RF_BP_VERSION 5u
+#define RF_BP_VERSION 6u
 #define RF_BP_FCHUNK_HDR 6u
 #define RF_BP_HDR_WORDS 18u
 
@@ -33,7 +33,7 @@ struct rf_bp_params {
     uint32_t rb[RF_BP_MAXR], re[RF_BP_MAXR];
     const uint32_t* prog;     // device copy of the blob
     uint32_t nslots;
-    uint32_t n_in, n_state, n_commit, n_hash, n_f;
+    uint32_t n_in, n_state, n_commit[2], n_hash, n_f;
     uint32_t n_state1;          // leading single-bit entries of the state list
     uint32_t off_in, off_state, off_commit[2], off_hash[2], off_f;
     uint32_t nlev[3], off_lev[3], off_gates[3];

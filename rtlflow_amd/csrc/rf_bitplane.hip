@@ -164,14 +164,13 @@ __device__ __forceinline__ uint64_t bp_hash(const uint64_t* L, const rf_bp_param
     return h;
 }
 
-// Commit: state <- next state for active lanes (the clock edge).
+// Commit: state <- next state (the clock edge) for the state bits the
+// program could not write in place. Unmasked: a lane that is not active
+// (done, or past the last stimulus) never writes its state or hash back, so
+// its planes may change freely during the launch.
 template <int NT>
-__device__ __forceinline__ void bp_commit(uint64_t* L, const uint32_t* c, uint32_t n, uint64_t am,
-                                          uint32_t tid) {
-    for (uint32_t i = tid; i < n; i += NT) {
-        const uint32_t st = c[2 * i], dn = c[2 * i + 1];
-        L[st] = (L[dn] & am) | (L[st] & ~am);
-    }
+__device__ __forceinline__ void bp_commit(uint64_t* L, const uint32_t* c, uint32_t n, uint32_t tid) {
+    for (uint32_t i = tid; i < n; i += NT) L[c[2 * i]] = L[c[2 * i + 1]];
 }
 
 // Rounds [i0, i1) of the register-resident gate records, a barrier where
@@ -210,7 +209,6 @@ template <int NT, int MAXR>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu<MAXR>())))
 rf_bp_kernel(rf_bp_params P) {
     extern __shared__ uint64_t L[];
-    __shared__ uint64_t s_act;
     const rf_args& a = P.a;
     const uint32_t tid = threadIdx.x;
     const uint32_t t = tid & 63u;
@@ -229,13 +227,7 @@ rf_bp_kernel(rf_bp_params P) {
     }
     constexpr uint32_t NW = NT / 64;
     const uint32_t wave = tid >> 6;
-    if (w0) {
-        const uint64_t am = __ballot((int)act);
-        if (t == 0) {
-            s_act = am;
-            L[0] = 0;
-        }
-    }
+    if (tid == 0) L[0] = 0;
     // state planes from the lane-minor arrays, spread over all wavefronts
     // (lane t of the group is thread t of every wavefront): single-bit
     // variables by one ballot each, wider ones by transposes
@@ -285,7 +277,6 @@ rf_bp_kernel(rf_bp_params P) {
         lev[k] = prog + P.off_lev[k];
         gates[k] = reinterpret_cast<const uint2*>(prog + P.off_gates[k]);
     }
-    const uint64_t am = s_act;
     constexpr int MR = MAXR > 0 ? MAXR : 1;
     uint2 d[MR];
     if constexpr (MAXR > 0) {
@@ -305,7 +296,7 @@ rf_bp_kernel(rf_bp_params P) {
         // (the commit writes state slots only; the lane work touches input
         // and output-copy slots only: one barrier covers both)
         bp_phase<NT>(L, lev[RF_BP_A0], gates[RF_BP_A0], P.nlev[RF_BP_A0], tid);
-        bp_commit<NT>(L, prog + P.off_commit[0], P.n_commit, am, tid);
+        bp_commit<NT>(L, prog + P.off_commit[0], P.n_commit[0], tid);
         if (w0) {
             bp_copy_inputs(L, P, t);
             if (N > 1) bp_inputs(L, P, sid, cyc + 1, t);
@@ -317,7 +308,7 @@ rf_bp_kernel(rf_bp_params P) {
                 bp_rounds<MR>(L, d, P, 0, (int)P.nr);
             else
                 bp_phase<NT>(L, lev[RF_BP_M], gates[RF_BP_M], P.nlev[RF_BP_M], tid);
-            bp_commit<NT>(L, prog + P.off_commit[1], P.n_commit, am, tid);
+            bp_commit<NT>(L, prog + P.off_commit[1], P.n_commit[1], tid);
             if (w1) {
                 const uint64_t hn = bp_hash(L, P, 0, h, t);
                 if (act) h = hn;
@@ -416,7 +407,8 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
     p->nslots = b[2];
     p->n_in = b[3];
     p->n_state = b[4];
-    p->n_commit = b[5];
+    p->n_commit[0] = b[5];
+    p->n_commit[1] = b[16];
     p->n_hash = b[6];
     p->n_f = b[7];
     uint32_t ngates[3];
@@ -439,7 +431,7 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
     for (int k = 0; k < 2; ++k) {
         o = ev(o);
         p->off_commit[k] = (uint32_t)o;
-        o += 2ull * p->n_commit;
+        o += 2ull * p->n_commit[k];
     }
     for (int k = 0; k < 3; ++k) {
         o = ev(o);
@@ -505,7 +497,7 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
             BP_FAIL("bit-plane state entry out of range");
     }
     for (int which = 0; which < 2; ++which)
-        for (uint32_t k = 0; k < p->n_commit; ++k) {
+        for (uint32_t k = 0; k < p->n_commit[which]; ++k) {
             const uint32_t* c = b + p->off_commit[which] + 2 * k;
             if (c[0] >= p->nslots || c[1] >= p->nslots) BP_FAIL("bit-plane commit slot out of range");
         }

@@ -59,8 +59,9 @@ class Program:
         s = self.sec
         self.inputs = b[s["in"]:s["in"] + 4 * self.n_in].reshape(-1, 4).astype(np.int64)
         self.state = b[s["state"]:s["state"] + 6 * self.n_state].reshape(-1, 6).astype(np.int64)
-        self.commits = [b[s[nm]:s[nm] + 2 * self.n_commit].reshape(-1, 2).astype(np.int64)
-                        for nm in ("commitA0", "commitM")]
+        ncm = (self.n_commit, blob[16])
+        self.commits = [b[s[nm]:s[nm] + 2 * ncm[k]].reshape(-1, 2).astype(np.int64)
+                        for k, nm in enumerate(("commitA0", "commitM"))]
         self.phases = [self._phase(s["lev" + nm], self.nlev[k], s["gates" + nm], self.ngates[k])
                        for k, nm in enumerate(("A0", "M", "B"))]
         self.hashes = []
@@ -113,8 +114,9 @@ class Program:
             L[sb:sb + nb] = L[sa:sa + nb]
 
     def commit(self, L, k, act):
+        # unmasked: lanes that are not active never write their state back
         st, dn = self.commits[k][:, 0], self.commits[k][:, 1]
-        L[st] = (L[dn] & act) | (L[st] & ~act)
+        L[st] = L[dn]
 
     def hash(self, L, k, h):
         for nbytes, width, refs in self.hashes[k]:
