@@ -8,6 +8,9 @@ Knobs:
   - ``threads``: the mtask fusion granularity (V3Partition's ``--threads``);
   - ``lane_major_min_depth``: the SoA layout of memories (lane-major region
     from this depth up, element-major below);
+  - ``inline_limit``: all state of a fused kernel in registers, or the
+    "calls" form whose mtask bodies work on the lane-minor arrays (for
+    designs whose state does not fit in registers);
 * run time:
   - the form (bit-plane, fused, plain graph, batched queue);
   - bit-plane workgroup size (256 / 512 / 1,024 threads per 64 lanes);
@@ -42,6 +45,9 @@ WORKLOADS = {
 
 COMPILE_SPACE = {
     "block": (64, 128, 256),
+    # 0: every design in the "calls" form (mtask bodies as functions over the
+    # lane-minor arrays) instead of all state in one kernel's registers
+    "inline_limit": (6000, 0),
 }
 MEM_LAYOUT = (64, 1 << 30)
 
@@ -67,9 +73,10 @@ def compile_candidates(name: str) -> List[Dict]:
     out = []
     for block in COMPILE_SPACE["block"]:
         for lm in (MEM_LAYOUT if has_mem else (64,)):
-            o = dict(base)
-            o.update(block=block, lane_major_min_depth=lm)
-            out.append(o)
+            for il in COMPILE_SPACE["inline_limit"]:
+                o = dict(base)
+                o.update(block=block, lane_major_min_depth=lm, inline_limit=il)
+                out.append(o)
     return out
 
 
