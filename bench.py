@@ -77,7 +77,8 @@ def main():
                     help="cycles of the learned-vs-heuristic partition comparison (0: skip)")
     ap.add_argument("--bp-threads", type=int, default=0,
                     help="bit-plane form: threads per 64-lane workgroup (0: runtime default)")
-    ap.add_argument("--workload", default="netlist", choices=["netlist", "alu", "systolic", "rv32i"],
+    ap.add_argument("--workload", default="netlist",
+                    choices=["netlist", "alu", "systolic", "rv32i", "net5s"],
                     help="netlist (default, the headline) or one of the other synthetic designs")
     ap.add_argument("--check", type=int, default=64, help="stimuli checked against the oracle")
     ap.add_argument("--cpu-sample", type=int, default=64, help="stimuli in the CPU baseline sample")
@@ -248,6 +249,8 @@ def bench_other(args):
     from rtlflow_amd.model import MODE_FUSED, MODE_GRAPH
     import oracle
     wl = args.workload
+    if wl == "net5s":
+        return bench_net5s(args)
     cfg = {"alu": 1, "rv32i": 3, "systolic": 4}[wl]
     seed = registry.SEED_CFG[cfg]
     cd = registry.get(wl)
@@ -347,6 +350,98 @@ def bench_other(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "integer", "data": "synthetic",
         "config": {"workload": workload, "design": wl, "form": main_form, "mtasks": man["mtasks"]},
         "forms": forms, "bit_exact_sample": all(v["bit_exact_sample"] for v in forms.values()),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_net5s(args):
+    """Random netlist with RAMs at one tenth of configuration 5's gate and
+    flip-flop counts: 4,096 stimuli with lognormal budgets through the
+    batched queue, partitioned by the instruction-count heuristic and by the
+    learned cost model, beside the plain per-mtask graph form at the median
+    budget."""
+    from rtlflow_amd.designs import registry, rv32i
+    from rtlflow_amd.model import MODE_BITPLANE, MODE_GRAPH
+    import oracle
+    seed = registry.SEED_CFG[5]
+    nstim = 4096
+    lanes = args.lanes if args.lanes != 65536 else nstim
+    budgets = rv32i.budgets(seed, nstim, median=2000, lo=200, hi=20000)
+    total = int(budgets.astype(np.int64).sum())
+    tb = registry.table_bytes("net5s")
+    nchk = min(args.check, nstim)
+    ref = oracle.netlist_run(tb, seed, nchk, budgets=budgets[:nchk], threads=16)
+    forms = {}
+    hashes = {}
+    for form, name in (("batched", "net5s"), ("batched_learned", "net5s_learned")):
+        cd = registry.get(name)
+        log(f"[bench] net5s {form}: {nstim} stimuli on {lanes} lanes, {total} stimulus-cycles, "
+            f"{cd.manifest['mtasks']} mtasks")
+        with cd.model(lanes) as m:
+            m.sync()
+            t0 = time.perf_counter()
+            out = m.submit_queue(seed, budgets)
+            dt = time.perf_counter() - t0
+        hashes[form] = out
+        ok = bool((out[:nchk] == ref).all())
+        forms[form] = {"seconds": dt, "value": total / dt, "bit_exact_sample": ok,
+                       "checked_stimuli": nchk, "lanes": lanes, "mtasks": cd.manifest["mtasks"]}
+        log(f"[bench] {form} {dt:.3f}s {total / dt:.4g} stim-cycles/s oracle_match={ok}")
+    same = bool((hashes["batched"] == hashes["batched_learned"]).all())
+    forms["batched_learned"]["same_hashes_as_heuristic"] = same
+    cd = registry.get("net5s")
+    if cd.manifest.get("bitplane", {}).get("supported"):
+        # the bit-plane form: stimuli dealt to 64-lane groups longest budget
+        # first, the RAMs as memory ports
+        nt = args.bp_threads or 1024
+        log(f"[bench] net5s bitplane: {nstim} stimuli on {lanes} lanes, {nt} threads per group")
+        with cd.model(lanes, bitplane_threads=nt) as m:
+            m.submit_queue(seed, budgets[:64], mode=MODE_BITPLANE)    # code load
+            m.sync()
+            t0 = time.perf_counter()
+            out = m.submit_queue(seed, budgets, mode=MODE_BITPLANE)
+            dt = time.perf_counter() - t0
+        ok = bool((out[:nchk] == ref).all())
+        forms["bitplane"] = {"seconds": dt, "value": total / dt, "bit_exact_sample": ok,
+                             "checked_stimuli": nchk, "lanes": lanes, "threads": nt,
+                             "same_hashes_as_batched": bool((out == hashes["batched"]).all())}
+        log(f"[bench] bitplane {dt:.3f}s {total / dt:.4g} stim-cycles/s oracle_match={ok}")
+    med = int(np.median(budgets))
+    cd = registry.get("net5s")
+    with cd.model(nstim) as m:
+        m.begin(seed)
+        m.sync()
+        t0 = time.perf_counter()
+        m.run_cycles(med, MODE_GRAPH)
+        m.sync()
+        dtg = time.perf_counter() - t0
+        hg = m.hashes()
+    refg = oracle.netlist_run(tb, seed, 8, cycles=med, threads=8)
+    forms["graph"] = {"seconds": dtg, "value": nstim * med / dtg,
+                      "bit_exact_sample": bool((hg[:8] == refg).all()), "checked_stimuli": 8,
+                      "lanes": nstim, "cycles": med}
+    log(f"[bench] graph {dtg:.3f}s {nstim * med / dtg:.4g} stim-cycles/s")
+    ns = args.cpu_sample
+    t0 = time.perf_counter()
+    oracle.netlist_run(tb, seed, ns, budgets=budgets[:ns], threads=16)
+    cdt = time.perf_counter() - t0
+    cpu = {"value": int(budgets[:ns].astype(np.int64).sum()) / cdt, "unit": "stimulus-cycles/s",
+           "cores": 16, "kind": "port", "sample": f"{ns} stimuli at their budgets, C oracle"}
+    exact = {k: v for k, v in forms.items() if v["bit_exact_sample"]}
+    main_form = max(exact or forms, key=lambda k: forms[k]["value"])
+    f = forms[main_form]
+    line = {
+        "metric": "stimulus-cycles/sec, bit-exact vs Verilator CPU; batched+learned vs plain GPU",
+        "value": f["value"], "unit": "stimulus-cycles/s", "n_gpus": 1, "steps": 1,
+        "warmup": args.warmup, "ms_per_step": f["seconds"] * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "integer", "data": "synthetic",
+        "config": {"workload": f"net5s: random netlist 50k gates / 5k flops / 8 x 1,024 x 64-bit RAMs / "
+                               f"128 PI / 64 PO, {nstim} stimuli, lognormal budgets (median 2000, "
+                               f"[200, 20000]), {total} cycles",
+                   "design": "net5s", "form": main_form},
+        "forms": forms, "bit_exact_sample": all(v["bit_exact_sample"] for v in forms.values()),
+        "learned_vs_heuristic": forms["batched_learned"]["value"] / forms["batched"]["value"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

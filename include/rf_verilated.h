@@ -108,6 +108,16 @@ int rf_hashes(rf_model* m, uint32_t lane0, uint32_t count, uint64_t* out);
 int rf_submit_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
                     const uint32_t* order, uint64_t* out_hashes);
 
+/* The batched workload in the bit-plane form (RF_E_STATE if the design has
+ * no bit-plane program or memory images are set): the stimuli are dealt to
+ * the lanes in queue order (order may be NULL: longest budget first), in
+ * rounds of n_lanes; every round starts from the reset state, and each
+ * 64-lane group runs as many cycles as its longest budget while every lane
+ * stops hashing at its own. out_hashes[nstim] as rf_submit_queue. Lane state
+ * afterwards is unspecified. No counterpart in the reference. */
+int rf_submit_queue_bitplane(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
+                             const uint32_t* order, uint64_t* out_hashes);
+
 /* Per-stimulus memory images for the batched form: before a stimulus runs,
  * elements [0, nelem) of memory ``mem`` in its lane are loaded from
  * src[stimulus][nelem][words]. Pass src = NULL to clear. */

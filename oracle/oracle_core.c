@@ -75,7 +75,7 @@ static int parse_table(const uint8_t* buf, size_t len, table_t* t) {
     t->ram = p;
     p += (size_t)t->n_rams * (t->abits + 1 + t->ram_width);
     if ((const uint8_t*)p > buf + len) return -2;
-    if (t->n_pi > 64 || t->n_po > 64 || (t->n_rams && t->ram_width > 64)) return -3;
+    if (t->n_pi > 128 || t->n_po > 64 || (t->n_rams && t->ram_width > 64)) return -3;
     return 0;
 }
 
@@ -101,14 +101,18 @@ static uint64_t run_one(const table_t* t, uint64_t seed, uint64_t s, uint32_t cy
     const uint32_t nodes = t->n_pi + t->n_ff + t->n_ram_bits + t->n_gates;
     memset(v, 0, nodes);
     if (t->n_rams) memset(ram, 0, (size_t)t->n_rams * t->ram_depth * 8);
-    const uint64_t pimask = t->n_pi >= 64 ? ~0ull : ((1ull << t->n_pi) - 1);
+    /* primary inputs: bits [64j, 64j + 64) of ``pi`` from draw k = j */
+    const uint32_t npiw = (t->n_pi + 63) / 64;
     const uint32_t obytes = t->n_po <= 8 ? 1 : t->n_po <= 16 ? 2 : t->n_po <= 32 ? 4 : 8;
     uint64_t h = OR_FNV_BASIS;
     const uint32_t rd0 = t->n_pi + t->n_ff;
     const uint64_t wmask = t->ram_width >= 64 ? ~0ull : ((1ull << t->ram_width) - 1);
     for (uint32_t c = 0; c < cycles; ++c) {
-        const uint64_t pi = or_stim(seed, s, c, 0) & pimask;
-        for (uint32_t i = 0; i < t->n_pi; ++i) v[i] = (uint8_t)((pi >> i) & 1u);
+        for (uint32_t j = 0; j < npiw; ++j) {
+            const uint64_t pi = or_stim(seed, s, c, j);
+            for (uint32_t i = 64 * j; i < t->n_pi && i < 64 * j + 64; ++i)
+                v[i] = (uint8_t)((pi >> (i - 64 * j)) & 1u);
+        }
         eval_comb(t, v);                                   /* clk = 0 eval */
         for (uint32_t i = 0; i < t->n_ff; ++i) dtmp[i] = v[t->ff_d[i]];
         for (uint32_t r = 0; r < t->n_rams; ++r) {         /* posedge: RAMs */

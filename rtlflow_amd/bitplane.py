@@ -25,10 +25,21 @@ runs them as one merged pass (program M) over two sets of input planes.
 Each phase keeps only the cone of what it produces and is levelised; the
 runtime kernel (csrc/rf_bitplane.hip) evaluates one level per barrier with
 every node value in LDS. Nodes take LDS slots by linear-scan allocation over
-levels. Anything outside the supported subset (memories, multiplies wider
-than 16 bits, division, combinational cycles, several clocks, negedge
-blocks, latches) raises ``Unsupported`` and the design keeps only the
-per-lane forms.
+levels.
+
+Synchronous memories stay in the lane-minor arrays (their per-lane layout is
+unchanged): a clocked block's ``x <= mem[addr]`` is a read port whose data
+planes the kernel writes straight into the state planes of ``x`` at the
+edge, and ``if (c) mem[addr] <= d`` is a write port with guard ``c``. At the
+edge each lane gathers its port addresses and data from the planes, loads
+the old word for every read port, then stores for every write port whose
+guard is set (non-blocking semantics).
+
+Anything outside the supported subset (asynchronous memory reads, memory
+read data used by logic before the edge, memories wider than 64 bits,
+multiplies wider than 16 bits, division, combinational cycles, several
+clocks, negedge blocks, latches) raises ``Unsupported`` and the design keeps
+only the per-lane forms.
 
 ``encode(prog)`` packs the program into the u32 blob the design library
 exports (``rf_design.bitplane``); its layout is documented on ``encode``.
@@ -50,7 +61,7 @@ class Unsupported(Exception):
     pass
 
 
-LEAVES = ("c0", "pi", "pib", "st")
+LEAVES = ("c0", "pi", "pib", "st", "mrd")
 
 
 class Net:
@@ -220,6 +231,12 @@ class Lowerer:
         self.pi: Dict[Var, List[Lit]] = {}
         self.pi_b: Dict[Var, List[Lit]] = {}
         self.state: Dict[Var, List[Lit]] = {}
+        # synchronous memory ports of the clocked blocks
+        self.in_seq = False
+        self.guard: Lit = T                  # path condition of the statement being run
+        self.rports: List[Tuple[Var, List[Lit], List[Lit]]] = []   # (mem, addr, data leaves)
+        self.rport_of: Dict[Tuple[int, Tuple[Lit, ...]], int] = {}
+        self.wports: List[Tuple[Var, Lit, List[Lit], List[Lit]]] = []  # (mem, guard, addr, data)
 
     # ----------------------------------------------------------- exprs
     def ev(self, e: Expr, env: Dict[Var, List[Lit]]) -> List[Lit]:
@@ -232,7 +249,17 @@ class Lowerer:
                 raise Unsupported(f"read of {e.var.name} before any value")
             return list(v)
         if isinstance(e, MemRd):
-            raise Unsupported("memory read")
+            if not self.in_seq:
+                raise Unsupported("asynchronous memory read")
+            m = e.var
+            self._check_mem(m)
+            addr = tuple(self._addr(e.idx, env))
+            key = (m.id, addr)
+            if key not in self.rport_of:
+                k = len(self.rports)
+                self.rport_of[key] = k
+                self.rports.append((m, list(addr), [n.leaf("mrd", (k, i)) for i in range(m.width)]))
+            return list(self.rports[self.rport_of[key]][2])
         if isinstance(e, Sel):
             a = self.ev(e.a, env)
             if isinstance(e.lsb, int):
@@ -331,6 +358,16 @@ class Lowerer:
         raise Unsupported(f"operator {o}")
 
     # ------------------------------------------------------- statements
+    def _check_mem(self, m: Var) -> None:
+        if m.width > 64:
+            raise Unsupported(f"memory {m.name} wider than 64 bits")
+
+    def _addr(self, idx: Expr, env) -> List[Lit]:
+        a = self.ev(idx, env)
+        if len(a) > 64:
+            raise Unsupported("memory index wider than 64 bits")
+        return a
+
     def write(self, env, lhs: Lhs, val: List[Lit], rd_env) -> None:
         v = lhs.var
         if v.is_mem or lhs.idx is not None:
@@ -379,7 +416,17 @@ class Lowerer:
         blocking writes); nba: pending non-blocking values (seq only)."""
         n = self.net
         for s in stmts:
-            if isinstance(s, Assign):
+            if isinstance(s, Assign) and s.lhs.var.is_mem:
+                m = s.lhs.var
+                if not (seq and s.nb):
+                    raise Unsupported(f"memory {m.name} written outside a clocked non-blocking assignment")
+                if s.lhs.lsb is not None:
+                    raise Unsupported(f"partial write of an element of {m.name}")
+                self._check_mem(m)
+                addr = self._addr(s.lhs.idx, env)
+                data = _fit(self.ev(s.rhs, env), m.width)
+                self.wports.append((m, self.guard, addr, data))
+            elif isinstance(s, Assign):
                 val = self.ev(s.rhs, env)
                 if s.nb:
                     if not seq:
@@ -399,10 +446,15 @@ class Lowerer:
                 raise Unsupported(f"statement {type(s).__name__}")
 
     def _branch(self, c, then, els, env, nba, seq):
+        n = self.net
+        g = self.guard
         e1, n1 = dict(env), dict(nba)
+        self.guard = n.AND(g, c)
         self.run(then, e1, n1, seq)
         e2, n2 = dict(env), dict(nba)
+        self.guard = n.AND(g, c ^ 1)
         self.run(els, e2, n2, seq)
+        self.guard = g
         env.update(self.merge(c, e1, e2, env))
         nba.update(self.merge(c, n1, n2, nba, keep_old=True))
 
@@ -419,10 +471,14 @@ class Lowerer:
             d = [n.XOR(b, T if (value >> i) & 1 else F) for i, b in enumerate(x) if (care >> i) & 1]
             hits.append(n.redor(d) ^ 1)
         c = n.redor(hits)
+        g = self.guard
         e1, n1 = dict(env), dict(nba)
+        self.guard = n.AND(g, c)
         self.run(body, e1, n1, seq)
         e2, n2 = dict(env), dict(nba)
+        self.guard = n.AND(g, c ^ 1)
         self._case(x, items, k + 1, e2, n2, seq)
+        self.guard = g
         env.update(self.merge(c, e1, e2, env))
         nba.update(self.merge(c, n1, n2, nba, keep_old=True))
 
@@ -454,7 +510,7 @@ class Lowerer:
                 self.run(p.stmts, local, {}, seq=False)
                 for v in p.writes:
                     if v.is_mem:
-                        raise Unsupported("memory")
+                        raise Unsupported("memory written by combinational logic")
                     if v not in local:
                         raise Unsupported(f"{v.name} not assigned")
                     env[v] = local[v]
@@ -469,8 +525,6 @@ class Lowerer:
             raise Unsupported("combinational cycle (settle loop)")
         if len(d.clocks) > 1:
             raise Unsupported("several clocks")
-        if any(v.is_mem for v in comp.stored):
-            raise Unsupported("memory")
         clk = d.clocks[0] if d.clocks else None
         seq_vx = [sch.vertices[vid] for vid in sch.topo if sch.vertices[vid].kind == "seq"]
         for vx in seq_vx:
@@ -510,7 +564,9 @@ class Lowerer:
                 for c in d.clocks:
                     env[c] = [T]
                 nba: Dict[Var, List[Lit]] = {}
+                self.in_seq, self.guard = True, T
                 self.run(p.stmts, env, nba, seq=True)
+                self.in_seq = False
                 for v in p.writes:
                     if v in nba:
                         nxt[v] = nba[v]
@@ -724,12 +780,42 @@ def compile_program(comp) -> Program:
         return copies[lit]
 
     # next-state roots: a literal that is a leaf or inverted gets a copy node
-    # so the commit reads a slot of its own
+    # so the commit reads a slot of its own. A state bit whose next value is
+    # a memory read port's data bit takes it from the port at the edge.
     commit_lits = []
+    rtargets: List[List[Tuple[int, int]]] = [[] for _ in lw.rports]
     for v in lw.state_vars:
         for i, lit in enumerate(lw.nxt[v]):
+            if net.kind[lit >> 1] == "mrd":
+                port, bit = net.info[lit >> 1]
+                if lit & 1:
+                    raise Unsupported("inverted memory read data")
+                rtargets[port].append((bit, fixed[lw.state[v][i] >> 1]))
+                continue
             commit_lits.append((lw.state[v][i] >> 1, lit))
     droots = [own(lit) if (lit & 1) or net.kind[lit >> 1] in LEAVES else lit for _, lit in commit_lits]
+
+    # memory port roots: the kernel gathers them at the edge, while the
+    # commit rewrites state planes and the next inputs are generated, so an
+    # input or state bit is copied
+    def port_lit(lit: Lit) -> Lit:
+        k = net.kind[lit >> 1]
+        if k == "mrd":
+            raise Unsupported("memory read data used by logic before the edge")
+        return own(lit & ~1) ^ (lit & 1) if k in ("pi", "pib", "st") else lit
+
+    mem_list: List[Var] = []
+    rp_lits = [[port_lit(b) for b in addr] for _, addr, _ in lw.rports]
+    wp_lits = [(port_lit(g), [port_lit(b) for b in addr], [port_lit(b) for b in data])
+               for _, g, addr, data in lw.wports]
+    for k, (m, _, _) in enumerate(lw.rports):
+        if rtargets[k] and m not in mem_list:
+            mem_list.append(m)
+    for m, _, _, _ in lw.wports:
+        if m not in mem_list:
+            mem_list.append(m)
+    proots = [b for k, lits in enumerate(rp_lits) if rtargets[k] for b in lits]
+    proots += [b for g, addr, data in wp_lits for b in [g] + addr + data]
     # hashed output roots: the hash of cycle n runs after the next commit and
     # while the inputs of later cycles are written, so an output bit that is
     # an input or state bit is copied
@@ -755,8 +841,34 @@ def compile_program(comp) -> Program:
             out.append((nbytes, v.width, [_lit_ref(b, slot) for b in hbits[v]]))
         return out
 
-    phA, slotA, peakA = build_phase(net, droots, fixed, nfixed, inplace=pairs)
-    phM, slotM, peakM = build_phase(net, droots + hroots, fixed, nfixed, inplace=pairs)
+    for x in _cone(net, droots + hroots + proots):
+        if any(net.kind[o >> 1] == "mrd" for o in net.ops[x]):
+            raise Unsupported("memory read data used by logic before the edge")
+
+    def ports_of(slot):
+        """Per memory (in mem_list order): its read ports, then its write
+        ports in statement order."""
+        recs = []
+        for mi, m in enumerate(mem_list):
+            for k, (mk, _, _) in enumerate(lw.rports):
+                if mk is m and rtargets[k]:
+                    recs.append((0, mi, [_lit_ref(b, slot) for b in rp_lits[k]], sorted(rtargets[k])))
+            for (mk, _, _, _), (g, addr, data) in zip(lw.wports, wp_lits):
+                if mk is m:
+                    recs.append((1, mi, [_lit_ref(b, slot) for b in addr], _lit_ref(g, slot),
+                                 [_lit_ref(b, slot) for b in data]))
+        return recs
+
+    design_mems = [v for v in d.vars if v.is_mem and lay.has(v)]
+    mem_table = []
+    for m in mem_list:
+        lm = m in lay.lane_major
+        mem_table.append((design_mems.index(m), CLS_ID[class_of(m)], 1 if lm else 0,
+                          lay.lane_major[m] if lm else lay.loc[m], m.depth * CLS_SIZE[m.cls],
+                          m.depth, m.width))
+
+    phA, slotA, peakA = build_phase(net, droots + proots, fixed, nfixed, inplace=pairs)
+    phM, slotM, peakM = build_phase(net, droots + hroots + proots, fixed, nfixed, inplace=pairs)
     phB, slotB, peakB = build_phase(net, hroots, fixed, nfixed)
     # phase F: every stored variable after the last cycle, in chunks whose
     # values all stay live until their write-back; chunks are sized so that
@@ -803,6 +915,8 @@ def compile_program(comp) -> Program:
     prog.phases = [phA, phM, phB]
     prog.commits = [commit_of(phA, slotA), commit_of(phM, slotM)]
     prog.hashes = [hash_of(slotM), hash_of(slotB)]
+    prog.mems = mem_table
+    prog.ports = [ports_of(slotA), ports_of(slotM)]
     prog.fchunks = fchunks
     prog.nodes = len(net.kind)
     prog.gates = [sum(len(l) for l in ph.levels) for ph in prog.phases] + \
@@ -832,7 +946,7 @@ def _final_bits(comp, lw: Lowerer, v: Var) -> Optional[List[Lit]]:
 # =====================================================================
 
 BP_MAGIC = 0x50424652   # "RFBP"
-BP_VERSION = 6
+BP_VERSION = 7
 HDR_WORDS = 18
 FCHUNK_HDR = 6
 
@@ -854,7 +968,8 @@ def encode(prog: Program) -> List[int]:
 
     header (18 words): magic, version, nslots, n_in, n_state, n_commit[A0],
     n_hash, n_fchunks, nlev[A0, M, B], ngates[A0, M, B], n_state1, total
-    words, n_commit[M], 0. The first n_state1 state entries are single-bit.
+    words, n_commit[M], n_mem. The first n_state1 state entries are
+    single-bit.
     Then, each section starting at an even word offset:
       inputs      n_in x {k, nbits, slot_a, slot_b}: bits of stimulus stream k
                   into slots slot_a ..; slot_b .. is the copy the clk = 1
@@ -869,6 +984,14 @@ def encode(prog: Program) -> List[int]:
                   dst | s << 16 | x << 32 | ix << 47 | y << 48 | iy << 63,
                   dst = s ? (x ^ ix) : (y ^ iy) on 64-lane planes
       hash M, hash B: n_hash x {nbytes, width, width slot refs} each
+      memories    n_mem x {design memory index, cls, layout (1: lane-major),
+                  loc_lo, loc_hi, row bytes, depth, width}
+      ports A0, ports M: {words (including these two), n_ports}, then per
+                  port, grouped by memory, reads before writes:
+                  read  {0, mem, nabits, ntarget, nabits address refs,
+                        ntarget x (data bit << 24 | state slot)}
+                  write {1, mem, nabits, width, guard ref, nabits address
+                        refs, width data refs}
       F chunks    n_fchunks x {nlev, ngates, n_wb, n_wb1, words, 0, level
                   offsets (padded to even), gates, write-back entries
                   {cls, loc_lo, loc_hi, words, width, width slot refs}, the
@@ -911,6 +1034,20 @@ def encode(prog: Program) -> List[int]:
         for nbytes, width, refs in hports:
             out += [nbytes, width] + refs
     pad(out)
+    for idx, cls, layout, loc, row, depth, width in prog.mems:
+        out += [idx, cls, layout, loc & 0xffffffff, loc >> 32, row, depth, width]
+    for recs in prog.ports:
+        pad(out)
+        body: List[int] = []
+        for r in recs:
+            if r[0] == 0:
+                _, mi, addr, targets = r
+                body += [0, mi, len(addr), len(targets)] + addr + [(b << 24) | st for b, st in targets]
+            else:
+                _, mi, addr, g, data = r
+                body += [1, mi, len(addr), len(data), g] + addr + data
+        out += [2 + len(body), len(recs)] + body
+    pad(out)
     for ph, entries in prog.fchunks:
         body = phase_words(ph) + _wb_words(entries)
         pad(body)
@@ -922,7 +1059,8 @@ def encode(prog: Program) -> List[int]:
            len(prog.commits[0]), len(prog.hashes[0]), len(prog.fchunks)]
     hdr += [len(ph.levels) for ph in prog.phases]
     hdr += [sum(len(l) for l in ph.levels) for ph in prog.phases]
-    hdr += [sum(1 for e in prog.state_load if e[3] == 1), len(out), len(prog.commits[1]), 0]
+    hdr += [sum(1 for e in prog.state_load if e[3] == 1), len(out), len(prog.commits[1]),
+            len(prog.mems)]
     out[:HDR_WORDS] = hdr
     return out
 
@@ -955,6 +1093,11 @@ def sections(blob: Sequence[int]) -> Dict[str, object]:
         res[name] = o = ev(o)
         for _ in range(n_hash):
             o += 2 + blob[o + 1]
+    res["mems"] = o = ev(o)
+    o += 8 * h[17]
+    for name in ("portsA0", "portsM"):
+        res[name] = o = ev(o)
+        o += blob[o]
     o = ev(o)
     chunks = []
     for _ in range(n_f):

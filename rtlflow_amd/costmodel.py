@@ -272,7 +272,7 @@ class LearnedCost:
 TRAIN_DESIGNS = ("net_small", "net2", "alu", "systolic", "rv32i", "fuzz1", "fuzz2", "fuzz3", "fuzz4",
                  "bfuzz1", "bfuzz2", "bfuzz3")
 TRAIN_THREADS = (2, 4, 8, 16)
-TARGETS = ("net2", "systolic", "rv32i")
+TARGETS = ("net2", "systolic", "rv32i", "net5s")
 PROFILE_LANES = 16384
 
 
@@ -398,7 +398,10 @@ def partition_candidates(names=TARGETS, threads=CANDIDATE_THREADS, log=print) ->
         out[name] = cands
     d = _cm_dir()
     d.mkdir(parents=True, exist_ok=True)
-    (d / "candidates.json").write_text(json.dumps(out))
+    path = d / "candidates.json"
+    merged = json.loads(path.read_text()) if path.exists() else {}
+    merged.update(out)
+    path.write_text(json.dumps(merged))
     return out
 
 
@@ -421,7 +424,8 @@ def main(argv=None):
     """GPU workflow. ``python -m rtlflow_amd.costmodel OUTDIR`` profiles the
     variants, trains, and predicts the targets' vertex costs and partition
     choices; ``python -m rtlflow_amd.costmodel OUTDIR --choose MODEL`` only
-    predicts the partition choices with a trained model."""
+    predicts the partition choices with a trained model; ``--candidates
+    PATH`` reads the candidates from PATH instead of the build folder."""
     import json
     import sys
     from pathlib import Path
@@ -430,6 +434,8 @@ def main(argv=None):
     out = Path(argv[0] if argv else "costmodel_out")
     out.mkdir(parents=True, exist_ok=True)
     cand_path = _cm_dir() / "candidates.json"
+    if "--candidates" in argv:
+        cand_path = Path(argv[argv.index("--candidates") + 1])
     if "--choose" in argv:
         model = json.loads(Path(argv[argv.index("--choose") + 1]).read_text())
     else:

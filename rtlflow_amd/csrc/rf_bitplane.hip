@@ -164,6 +164,77 @@ __device__ __forceinline__ uint64_t bp_hash(const uint64_t* L, const rf_bp_param
     return h;
 }
 
+// Memory table entry: {design memory, cls, layout, loc_lo, loc_hi, row
+// bytes, depth, width}; element e of lane ``lane`` lives where the per-lane
+// kernels keep it (element-major: slot loc + e of its class array;
+// lane-major: byte row ``lane`` of the region at loc * npad).
+__device__ __forceinline__ uint64_t bp_mem_load(const rf_args& a, const uint32_t* me, uint32_t lane,
+                                                uint64_t e) {
+    const uint64_t loc = (uint64_t)me[3] | ((uint64_t)me[4] << 32);
+    if (!me[2]) return bp_load(a, me[1], loc + e, lane);
+    const uint8_t* row = a.mem + loc * a.npad + (size_t)lane * me[5];
+    switch (me[1]) {
+        case RF_CLS_C: return row[e];
+        case RF_CLS_S: return reinterpret_cast<const uint16_t*>(row)[e];
+        case RF_CLS_I: return reinterpret_cast<const uint32_t*>(row)[e];
+        default: return reinterpret_cast<const uint64_t*>(row)[e];
+    }
+}
+
+__device__ __forceinline__ void bp_mem_store(const rf_args& a, const uint32_t* me, uint32_t lane, uint64_t e,
+                                             uint64_t v) {
+    const uint64_t loc = (uint64_t)me[3] | ((uint64_t)me[4] << 32);
+    if (!me[2]) {
+        bp_store(a, me[1], loc + e, lane, v);
+        return;
+    }
+    uint8_t* row = a.mem + loc * a.npad + (size_t)lane * me[5];
+    switch (me[1]) {
+        case RF_CLS_C: row[e] = (uint8_t)v; break;
+        case RF_CLS_S: reinterpret_cast<uint16_t*>(row)[e] = (uint16_t)v; break;
+        case RF_CLS_I: reinterpret_cast<uint32_t*>(row)[e] = (uint32_t)v; break;
+        default: reinterpret_cast<uint64_t*>(row)[e] = v; break;
+    }
+}
+
+// Memory ports at the clock edge, after program ``which`` (0: A0, 1: M).
+// Memory mi is served by wavefront (2 + mi) mod NW, all 64 threads of it
+// (lane t of the group is thread t): per port in list order (a memory's
+// reads before its writes) each lane gathers its address (and guard and
+// data) from the planes; a read loads the lane's old word and transposes
+// the words into the target state planes; a write stores when the guard is
+// set, the address is in range and the lane is active. Reads and writes of
+// one memory stay in one thread per lane, in program order.
+__device__ __forceinline__ void bp_mem_ports(uint64_t* L, const rf_bp_params& P, uint32_t which,
+                                             uint32_t wave, uint32_t nw, uint32_t t, uint32_t lane,
+                                             bool inb, bool act) {
+    const uint32_t* pp = P.prog + P.off_ports[which];
+    const uint32_t n = pp[1];
+    const uint32_t* mt = P.prog + P.off_mem;
+    pp += 2;
+    for (uint32_t r = 0; r < n; ++r) {
+        const uint32_t kind = pp[0], mi = pp[1], na = pp[2], n2 = pp[3];
+        if ((2u + mi) % nw == wave) {
+            const uint32_t* me = mt + RF_BP_MEM_WORDS * mi;
+            const uint32_t depth = me[6];
+            if (kind == 0) {
+                const uint64_t e = bp_gather(L, pp + 4, na, t);
+                const uint64_t v = (inb && e < depth) ? bp_mem_load(P.a, me, lane, e) : 0ull;
+                const uint64_t col = bp_transpose64(v, t);
+                const uint32_t* tg = pp + 4 + na;
+                for (uint32_t j = 0; j < n2; ++j)
+                    if ((tg[j] >> 24) == t) L[tg[j] & 0xffffffu] = col;
+            } else {
+                const bool g = (bp_ref(L, pp[4]) >> t) & 1ull;
+                const uint64_t e = bp_gather(L, pp + 5, na, t);
+                const uint64_t v = bp_gather(L, pp + 5 + na, n2, t);
+                if (act && g && e < depth) bp_mem_store(P.a, me, lane, e, v);
+            }
+        }
+        pp += (kind == 0 ? 4u : 5u) + na + n2;
+    }
+}
+
 // Commit: state <- next state (the clock edge) for the state bits the
 // program could not write in place. Unmasked: a lane that is not active
 // (done, or past the last stimulus) never writes its state or hash back, so
@@ -201,12 +272,13 @@ __device__ __forceinline__ void bp_rounds(uint64_t* L, const uint2 (&d)[MAXR], c
 // no global-memory loads; MAXR = 0: records are streamed from memory.
 // Up to 24 register-resident rounds the kernel fits in 80 registers, which
 // leaves room for 6 wavefronts per SIMD (3 workgroups of 512 threads per CU
-// when the LDS allows).
-template <int MAXR>
-constexpr int bp_waves_per_eu() { return MAXR > 0 && MAXR <= 24 ? 6 : 1; }
+// when the LDS allows). A 1,024-thread workgroup needs 4 wavefronts per SIMD
+// (at most 128 registers each), which holds up to 40 rounds.
+template <int NT, int MAXR>
+constexpr int bp_waves_per_eu() { return NT >= 1024 ? 4 : (MAXR > 0 && MAXR <= 24 ? 6 : 1); }
 
 template <int NT, int MAXR>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu<MAXR>())))
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu<NT, MAXR>())))
 rf_bp_kernel(rf_bp_params P) {
     extern __shared__ uint64_t L[];
     const rf_args& a = P.a;
@@ -227,6 +299,21 @@ rf_bp_kernel(rf_bp_params P) {
     }
     constexpr uint32_t NW = NT / 64;
     const uint32_t wave = tid >> 6;
+    // cycles this lane advances: a.ncycles, or with per-stimulus budgets
+    // (a.budget) what is left of its budget; the group runs as many cycles
+    // as its longest lane needs
+    const bool bud = a.budget != nullptr;
+    uint32_t lim = a.ncycles;
+    __shared__ uint32_t s_ncycles;
+    if (bud && (w0 || w1)) {
+        const uint32_t b = act ? a.budget[sid] : 0u;
+        lim = b > cyc ? min(a.ncycles, b - cyc) : 0u;
+        if (w0) {
+            uint32_t mx = lim;
+            for (uint32_t k = 32; k; k >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, (int)k, 64));
+            if (t == 0) s_ncycles = mx;
+        }
+    }
     if (tid == 0) L[0] = 0;
     // state planes from the lane-minor arrays, spread over all wavefronts
     // (lane t of the group is thread t of every wavefront): single-bit
@@ -290,13 +377,14 @@ rf_bp_kernel(rf_bp_params P) {
             }
         }
     }
-    const uint32_t N = a.ncycles;
+    const uint32_t N = bud ? s_ncycles : a.ncycles;
     if (N) {
         // cycle 0's next state
         // (the commit writes state slots only; the lane work touches input
         // and output-copy slots only: one barrier covers both)
         bp_phase<NT>(L, lev[RF_BP_A0], gates[RF_BP_A0], P.nlev[RF_BP_A0], tid);
         bp_commit<NT>(L, prog + P.off_commit[0], P.n_commit[0], tid);
+        if (P.n_mem) bp_mem_ports(L, P, 0, wave, NW, t, lane, inb, act);
         if (w0) {
             bp_copy_inputs(L, P, t);
             if (N > 1) bp_inputs(L, P, sid, cyc + 1, t);
@@ -309,9 +397,10 @@ rf_bp_kernel(rf_bp_params P) {
             else
                 bp_phase<NT>(L, lev[RF_BP_M], gates[RF_BP_M], P.nlev[RF_BP_M], tid);
             bp_commit<NT>(L, prog + P.off_commit[1], P.n_commit[1], tid);
+            if (P.n_mem) bp_mem_ports(L, P, 1, wave, NW, t, lane, inb, act);
             if (w1) {
                 const uint64_t hn = bp_hash(L, P, 0, h, t);
-                if (act) h = hn;
+                if (act && n < lim) h = hn;
             } else if (w0) {
                 bp_copy_inputs(L, P, t);
                 if (n + 2 < N) bp_inputs(L, P, sid, cyc + n + 2, t);
@@ -322,7 +411,7 @@ rf_bp_kernel(rf_bp_params P) {
         bp_phase<NT>(L, lev[RF_BP_B], gates[RF_BP_B], P.nlev[RF_BP_B], tid);
         if (w1) {
             const uint64_t hn = bp_hash(L, P, 1, h, t);
-            if (act) h = hn;
+            if (act && N - 1 < lim) h = hn;
         }
         __syncthreads();
     }
@@ -330,9 +419,12 @@ rf_bp_kernel(rf_bp_params P) {
     // arrays chunk by chunk (a chunk's values stay in LDS until written);
     // the write-back is spread over all wavefronts: single-bit variables by
     // one broadcast read each, wider ones by transposes (every lane of a
-    // wavefront takes part; only active lanes store)
+    // wavefront takes part; only active lanes store). With budgets the lanes
+    // stopped at different cycles: no state is written back (the hashes are
+    // the result).
     const uint32_t* fc = prog + P.off_f;
-    for (uint32_t c = 0; c < P.n_f; ++c) {
+    const uint32_t nf = bud ? 0u : P.n_f;
+    for (uint32_t c = 0; c < nf; ++c) {
         const uint32_t fnl = fc[0], fng = fc[1], nwb = fc[2], nwb1 = fc[3];
         const uint32_t* flev = fc + RF_BP_FCHUNK_HDR;
         uint32_t go = RF_BP_FCHUNK_HDR + fnl + 1;
@@ -368,7 +460,8 @@ rf_bp_kernel(rf_bp_params P) {
     }
     if (w1 && act) {
         a.hash[lane] = h;
-        a.cyc[lane] = cyc + a.ncycles;
+        a.cyc[lane] = cyc + lim;
+        if (bud) a.out_hash[sid] = h;
     }
 }
 
@@ -460,6 +553,45 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
         }
     }
     o = ev(o);
+    p->n_mem = b[17];
+    p->off_mem = (uint32_t)o;
+    o += (uint64_t)RF_BP_MEM_WORDS * p->n_mem;
+    if (o > words) BP_FAIL("bit-plane memory table truncated");
+    for (uint32_t k = 0; k < p->n_mem; ++k) {
+        const uint32_t* me = b + p->off_mem + RF_BP_MEM_WORDS * k;
+        if (me[1] > 3 || me[2] > 1 || me[6] == 0 || me[7] == 0 || me[7] > 64 ||
+            me[7] > 8u * (1u << me[1]) || (me[2] && (uint64_t)me[6] << me[1] > me[5]))
+            BP_FAIL("bit-plane memory entry malformed");
+    }
+    for (int which = 0; which < 2; ++which) {
+        o = ev(o);
+        p->off_ports[which] = (uint32_t)o;
+        if (o + 2 > words || b[o] < 2 || o + b[o] > words) BP_FAIL("bit-plane port list malformed");
+        const uint64_t end = o + b[o];
+        uint64_t q = o + 2;
+        for (uint32_t r = 0; r < b[o + 1]; ++r) {
+            if (q + 5 > end) BP_FAIL("bit-plane port list truncated");
+            const uint32_t kind = b[q], mi = b[q + 1], na = b[q + 2], n2 = b[q + 3];
+            const uint64_t len = (kind == 0 ? 4u : 5u) + (uint64_t)na + n2;
+            if (kind > 1 || mi >= p->n_mem || na == 0 || na > 64 || n2 > 64 || q + len > end)
+                BP_FAIL("bit-plane port malformed");
+            if (kind == 0) {
+                if (!bp_check_refs(b + q + 4, na, p->nslots)) BP_FAIL("bit-plane port slot out of range");
+                for (uint32_t j = 0; j < n2; ++j) {
+                    const uint32_t tg = b[q + 4 + na + j];
+                    if ((tg >> 24) >= b[p->off_mem + RF_BP_MEM_WORDS * mi + 7] || (tg & 0xffffffu) >= p->nslots)
+                        BP_FAIL("bit-plane read port target out of range");
+                }
+            } else {
+                if (n2 != b[p->off_mem + RF_BP_MEM_WORDS * mi + 7] || !bp_check_refs(b + q + 4, 1 + na + n2, p->nslots))
+                    BP_FAIL("bit-plane write port malformed");
+            }
+            q += len;
+        }
+        if (q != end) BP_FAIL("bit-plane port list size mismatch");
+        o = end;
+    }
+    o = ev(o);
     p->off_f = (uint32_t)o;
     for (uint32_t c = 0; c < p->n_f; ++c) {
         if (o + RF_BP_FCHUNK_HDR > words) BP_FAIL("bit-plane write-back chunk truncated");
@@ -545,12 +677,12 @@ static hipError_t bp_launch_k(const rf_bp_params& p, hipStream_t stream) {
 template <int NT>
 static hipError_t bp_launch_nt(const rf_bp_params& p, hipStream_t stream) {
     const uint32_t r = p.nr;
-    if (r == 0 || r > RF_BP_MAXR) return bp_launch_k<NT, 0>(p, stream);
+    if (r == 0 || r > RF_BP_MAXR || (NT >= 1024 && r > 40)) return bp_launch_k<NT, 0>(p, stream);
     if (r <= 16) return bp_launch_k<NT, 16>(p, stream);
     if (r <= 24) return bp_launch_k<NT, 24>(p, stream);
     if (r <= 32) return bp_launch_k<NT, 32>(p, stream);
-    if (r <= 40) return bp_launch_k<NT, 40>(p, stream);
-    return bp_launch_k<NT, 48>(p, stream);
+    if (NT >= 1024 || r <= 40) return bp_launch_k<NT, 40>(p, stream);
+    return bp_launch_k<NT, (NT >= 1024 ? 40 : 48)>(p, stream);
 }
 
 hipError_t rf_bp_launch(const rf_bp_params& p, uint32_t threads, hipStream_t stream) {
@@ -558,7 +690,7 @@ hipError_t rf_bp_launch(const rf_bp_params& p, uint32_t threads, hipStream_t str
         case 128: return bp_launch_k<128, 0>(p, stream);
         case 256: return bp_launch_nt<256>(p, stream);
         case 512: return bp_launch_nt<512>(p, stream);
-        case 1024: return bp_launch_k<1024, 0>(p, stream);
+        case 1024: return bp_launch_nt<1024>(p, stream);
         default: return hipErrorInvalidValue;
     }
 }

@@ -208,6 +208,18 @@ int load_bitplane(rf_model* m) {
         }
         q += b[q + 4];
     }
+    // memories: the blob's table must describe the design's own layout
+    for (uint32_t k = 0; k < p.n_mem; ++k) {
+        const uint32_t* me = b + p.off_mem + RF_BP_MEM_WORDS * k;
+        if (me[0] >= d->nmems) return fail(RF_E_DESIGN, "bit-plane memory outside the design");
+        const rf_mem& md = d->mems[me[0]];
+        const uint64_t loc = (uint64_t)me[3] | ((uint64_t)me[4] << 32);
+        if (md.cls != me[1] || md.layout != me[2] || md.loc != loc || md.depth != me[6] ||
+            md.width != me[7] || md.words != 1 || (me[2] && md.row_bytes != me[5]))
+            return fail(RF_E_DESIGN, "bit-plane memory %s does not match the design layout", md.name);
+        if (me[2] ? loc + md.row_bytes > d->mem_row_bytes : loc + md.depth > d->slots[md.cls])
+            return fail(RF_E_DESIGN, "bit-plane memory %s outside the layout", md.name);
+    }
     const size_t bytes = d->bitplane_words * 4;
     RF_HIP(hipMalloc(&m->bp_prog, bytes));
     RF_HIP(hipMemcpy(m->bp_prog, b, bytes, hipMemcpyHostToDevice));
@@ -619,10 +631,10 @@ int rf_set_mem_images(rf_model* m, uint32_t mi, uint32_t nstim, uint32_t nelem, 
     return RF_OK;
 }
 
-int rf_submit_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
-                    const uint32_t* order, uint64_t* out_hashes) {
-    if (!m || !budgets || !out_hashes) return fail(RF_E_ARG, "null argument");
-    if (!nstim) return RF_OK;
+// Queue buffers for nstim stimuli and the queue order (the caller's, or
+// longest budget first).
+static int queue_setup(rf_model* m, const uint32_t* budgets, uint32_t nstim, const uint32_t* order,
+                std::vector<uint32_t>& ord) {
     if (nstim > m->qcap) {
         if (m->budget) (void)hipFree(m->budget);
         if (m->order) (void)hipFree(m->order);
@@ -630,12 +642,13 @@ int rf_submit_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_
         m->budget = nullptr;
         m->order = nullptr;
         m->out_hash = nullptr;
+        m->qcap = 0;
         RF_HIP(hipMalloc(&m->budget, (size_t)nstim * 4));
         RF_HIP(hipMalloc(&m->order, (size_t)nstim * 4));
         RF_HIP(hipMalloc(&m->out_hash, (size_t)nstim * 8));
         m->qcap = nstim;
     }
-    std::vector<uint32_t> ord(nstim);
+    ord.resize(nstim);
     if (order) {
         std::vector<char> seen(nstim, 0);
         for (uint32_t k = 0; k < nstim; ++k) {
@@ -649,6 +662,16 @@ int rf_submit_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_
         std::stable_sort(ord.begin(), ord.end(),
                          [&](uint32_t x, uint32_t y) { return budgets[x] > budgets[y]; });
     }
+    return RF_OK;
+}
+
+int rf_submit_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
+                    const uint32_t* order, uint64_t* out_hashes) {
+    if (!m || !budgets || !out_hashes) return fail(RF_E_ARG, "null argument");
+    if (!nstim) return RF_OK;
+    std::vector<uint32_t> ord;
+    int qrc = queue_setup(m, budgets, nstim, order, ord);
+    if (qrc) return qrc;
     RF_HIP(hipMemcpyAsync(m->budget, budgets, (size_t)nstim * 4, hipMemcpyHostToDevice, m->stream));
     RF_HIP(hipMemcpyAsync(m->order, ord.data(), (size_t)nstim * 4, hipMemcpyHostToDevice, m->stream));
     RF_HIP(hipMemsetAsync(m->out_hash, 0, (size_t)nstim * 8, m->stream));
@@ -674,6 +697,71 @@ int rf_submit_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_
     RF_HIP(hipStreamSynchronize(m->stream));
     m->did_init = true;
     return check_flags(m);
+}
+
+int rf_submit_queue_bitplane(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
+                             const uint32_t* order, uint64_t* out_hashes) {
+    if (!m || !budgets || !out_hashes) return fail(RF_E_ARG, "null argument");
+    if (!m->bp_prog) return fail(RF_E_STATE, "design %s has no bit-plane program", m->d->name);
+    if (m->base.images)
+        return fail(RF_E_STATE, "memory images are loaded by the per-lane batched form only");
+    if (!nstim) return RF_OK;
+    std::vector<uint32_t> ord;
+    int rc = queue_setup(m, budgets, nstim, order, ord);
+    if (rc) return rc;
+    RF_HIP(hipMemcpyAsync(m->budget, budgets, (size_t)nstim * 4, hipMemcpyHostToDevice, m->stream));
+    RF_HIP(hipMemsetAsync(m->out_hash, 0, (size_t)nstim * 8, m->stream));
+    const uint32_t nl = m->nlanes;
+    std::vector<uint64_t> h(m->npad, 0xcbf29ce484222325ull);
+    std::vector<uint32_t> sid(m->npad, 0);
+    std::vector<uint8_t> done(m->npad, 1);
+    for (uint32_t r0 = 0; r0 < nstim; r0 += nl) {
+        const uint32_t cnt = std::min(nl, nstim - r0);
+        uint32_t ncyc = 0;
+        for (uint32_t k = 0; k < nl; ++k) {
+            done[k] = k < cnt ? 0 : 1;
+            sid[k] = k < cnt ? ord[r0 + k] : 0;
+            if (k < cnt) ncyc = std::max(ncyc, budgets[ord[r0 + k]]);
+        }
+        // every lane starts from the reset state: zeroed arrays and memories,
+        // then the initial blocks
+        for (int c = 0; c < 4; ++c)
+            if (m->d->slots[c])
+                RF_HIP(hipMemsetAsync(m->cls[c], 0, m->d->slots[c] * m->npad * kClassBytes[c], m->stream));
+        if (m->mem) RF_HIP(hipMemsetAsync(m->mem, 0, m->d->mem_row_bytes * m->npad, m->stream));
+        m->did_init = false;
+        rc = do_init(m);
+        if (rc) return rc;
+        RF_HIP(hipMemcpyAsync(m->hash, h.data(), m->npad * 8, hipMemcpyHostToDevice, m->stream));
+        RF_HIP(hipMemcpyAsync(m->stim, sid.data(), m->npad * 4, hipMemcpyHostToDevice, m->stream));
+        RF_HIP(hipMemcpyAsync(m->done, done.data(), m->npad, hipMemcpyHostToDevice, m->stream));
+        RF_HIP(hipMemsetAsync(m->cyc, 0, m->npad * 4, m->stream));
+        if (!ncyc) continue;
+        rf_bp_params p = m->bp;
+        rf_bp_rounds(m->d->bitplane, m->cfg.bitplane_threads, &p);
+        p.a = args_for(m, RF_PHASE_EVAL);
+        p.a.seed = seed;
+        p.a.ncycles = ncyc;
+        p.a.budget = m->budget;
+        p.a.out_hash = m->out_hash;
+        p.a.nstim = nstim;
+        RF_HIP(hipEventRecord(m->ev0, m->stream));
+        RF_HIP(rf_bp_launch(p, m->cfg.bitplane_threads, m->stream));
+        RF_HIP(hipEventRecord(m->ev1, m->stream));
+        RF_HIP(hipEventSynchronize(m->ev1));
+        float ms = 0;
+        RF_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+        m->stats.bitplane_launches++;
+        m->stats.bitplane_ms += ms;
+    }
+    // stimuli of budget 0 keep the FNV offset basis
+    std::vector<uint64_t> oh(nstim);
+    RF_HIP(hipMemcpyAsync(oh.data(), m->out_hash, (size_t)nstim * 8, hipMemcpyDeviceToHost, m->stream));
+    std::fill(done.begin(), done.end(), 0);
+    RF_HIP(hipMemcpyAsync(m->done, done.data(), m->npad, hipMemcpyHostToDevice, m->stream));
+    RF_HIP(hipStreamSynchronize(m->stream));
+    for (uint32_t k = 0; k < nstim; ++k) out_hashes[k] = budgets[k] ? oh[k] : 0xcbf29ce484222325ull;
+    return RF_OK;
 }
 
 int rf_lane_state(rf_model* m, uint32_t lane, void* dst, size_t* bytes) {

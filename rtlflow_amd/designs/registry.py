@@ -71,6 +71,11 @@ register("net_small", _netlist(n_gates=2000, n_ff=200, n_pi=64, n_po=64, depth=2
                                seed=SEED_CFG[2] ^ 0xabc, name="net_small"), threads=4)
 # configuration 2: 20,000 gates, 2,000 flip-flops, 64 PI / 64 PO, depth 40
 register("net2", _netlist(seed=SEED_CFG[2], name="net2"), threads=4)
+# configuration 5 at one tenth of its gate and flip-flop counts (50,000 gates,
+# 5,000 flip-flops) with its full RAM complement: eight 1,024 x 64-bit
+# synchronous RAMs, 128 primary inputs, 64 primary outputs
+register("net5s", _netlist(n_gates=50000, n_ff=5000, n_pi=128, n_po=64, depth=40, seed=SEED_CFG[5],
+                           n_rams=8, ram_depth=1024, ram_width=64, name="net5s"), threads=4)
 # configuration 1: 4-stage pipelined ALU with a 32 x 32 register file
 register("alu", _alu(), threads=2)
 # configuration 4: 16 x 16 int8 systolic array + 4,096-bit shift/compare chain
@@ -178,6 +183,7 @@ def _fuzz(seed: int):
 register("net2_learned", _netlist(seed=SEED_CFG[2], name="net2"), learned_threads="net2",
          bitplane=False)
 register("systolic_learned", _systolic(), learned_threads="systolic", bitplane=False)
+register("net5s_learned", ENTRIES["net5s"].make, learned_threads="net5s", bitplane=False)
 register("rv32i_learned", _rv32i(), learned_threads="rv32i", bitplane=False)
 
 # the autotuner's choice of compile-time options, where it measured one
@@ -210,6 +216,21 @@ def _bfuzz(seed: int):
                 "top": f"bfuzz{seed}"}
     return make
 
+
+# synchronous memories in the bit-plane form: a RAM netlist (lane-major
+# 9-bit RAMs) and the memory-port coverage design
+register("net_ram", _netlist(n_gates=2000, n_ff=200, n_pi=70, n_po=40, depth=12, seed=SEED_CFG[5] ^ 0x77,
+                             n_rams=4, ram_depth=100, ram_width=9, name="net_ram"), threads=3)
+
+
+def _rams():
+    def make():
+        from .rams import verilog
+        return {"verilog": verilog(), "top": "rams"}
+    return make
+
+
+register("rams", _rams(), threads=2)
 
 # random designs inside the bit-plane subset (no memories, no division)
 for _s in (1, 2, 3):

@@ -69,6 +69,7 @@ def runtime() -> C.CDLL:
         "rf_run_cycles": (C.c_int, [P, u32, u32]),
         "rf_hashes": (C.c_int, [P, u32, u32, P]),
         "rf_submit_queue": (C.c_int, [P, u64, P, u32, P, P]),
+        "rf_submit_queue_bitplane": (C.c_int, [P, u64, P, u32, P, P]),
         "rf_set_mem_images": (C.c_int, [P, u32, u32, u32, P]),
         "rf_lane_state": (C.c_int, [P, u32, P, C.POINTER(C.c_size_t)]),
         "rf_profile_mtasks": (C.c_int, [P, u32, P]),
@@ -218,15 +219,24 @@ class Model:
         _check(self.rt.rf_hashes(self.h, 0, self.n, out.ctypes.data))
         return out
 
-    def submit_queue(self, seed: int, budgets: Sequence[int], order=None) -> np.ndarray:
+    def submit_queue(self, seed: int, budgets: Sequence[int], order=None,
+                     mode: int = MODE_FUSED) -> np.ndarray:
+        """Per-stimulus hashes of a batched workload: ``mode`` MODE_FUSED runs
+        the persistent per-lane kernel with its work queue, MODE_BITPLANE the
+        bit-plane kernel over rounds of lanes."""
         b = np.ascontiguousarray(budgets, dtype=np.uint32)
         out = np.zeros(len(b), dtype=np.uint64)
         optr = None
         if order is not None:
             o = np.ascontiguousarray(order, dtype=np.uint32)
             optr = o.ctypes.data
-        _check(self.rt.rf_submit_queue(self.h, seed & ((1 << 64) - 1), b.ctypes.data, len(b), optr,
-                                       out.ctypes.data))
+        if mode == MODE_BITPLANE:
+            fn = self.rt.rf_submit_queue_bitplane
+        elif mode == MODE_FUSED:
+            fn = self.rt.rf_submit_queue
+        else:
+            raise ValueError(f"no batched workload in mode {mode}")
+        _check(fn(self.h, seed & ((1 << 64) - 1), b.ctypes.data, len(b), optr, out.ctypes.data))
         return out
 
     def lane_state(self, lane: int) -> bytes:

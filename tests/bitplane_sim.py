@@ -2,8 +2,10 @@
 
 Mirrors what the runtime's bit-plane kernel does with the blob -- input
 planes from the stimulus generator, levelised gates over 64-lane planes,
-masked commit, per-lane output hashing, write-back -- so the lowering,
-scheduling and encoding can be checked on the CPU against the oracle."""
+unmasked commit, synchronous memory ports at the edge (per-lane reads of the
+old words, then guarded writes), per-lane output hashing, write-back -- so
+the lowering, scheduling and encoding can be checked on the CPU against the
+oracle."""
 import numpy as np
 
 from rtlflow_amd import bitplane as bp
@@ -73,6 +75,28 @@ class Program:
                 hp.append((nbytes, width, b[o + 2:o + 2 + width].astype(np.int64)))
                 o += 2 + width
             self.hashes.append(hp)
+        self.mems = [tuple(int(x) for x in b[s["mems"] + 8 * i:s["mems"] + 8 * i + 8])
+                     for i in range(blob[17])]
+        self.ports = []
+        for nm in ("portsA0", "portsM"):
+            o = s[nm]
+            nrec = int(b[o + 1])
+            o += 2
+            recs = []
+            for _ in range(nrec):
+                kind, mi, na, n2 = (int(x) for x in b[o:o + 4])
+                if kind == 0:
+                    addr = b[o + 4:o + 4 + na].astype(np.int64)
+                    tg = [(int(x) >> 24, int(x) & 0xFFFFFF) for x in b[o + 4 + na:o + 4 + na + n2]]
+                    recs.append((0, mi, addr, tg))
+                    o += 4 + na + n2
+                else:
+                    g = int(b[o + 4])
+                    addr = b[o + 5:o + 5 + na].astype(np.int64)
+                    data = b[o + 5 + na:o + 5 + na + n2].astype(np.int64)
+                    recs.append((1, mi, addr, g, data))
+                    o += 5 + na + n2
+            self.ports.append(recs)
         self.fchunks = []
         for c in s["fchunks"]:
             ph = self._phase(c["lev"], c["nlev"], c["gates"], c["ngates"])
@@ -118,6 +142,33 @@ class Program:
         st, dn = self.commits[k][:, 0], self.commits[k][:, 1]
         L[st] = L[dn]
 
+    def new_mems(self):
+        """Zeroed memories of one 64-lane group: [64, depth] per memory."""
+        return [np.zeros((64, m[6]), np.uint64) for m in self.mems]
+
+    def lane_vals(self, L, refs):
+        return from_planes(np.array([self.ref(L, int(r)) for r in refs], np.uint64))
+
+    def ram_step(self, L, k, act, mem):
+        lanes = np.arange(64)
+        on = ((np.uint64(act) >> lanes.astype(np.uint64)) & np.uint64(1)) == 1
+        for rec in self.ports[k]:
+            mi = rec[1]
+            depth = self.mems[mi][6]
+            m = mem[mi]
+            addr = self.lane_vals(L, rec[2]).astype(np.int64)
+            ok = addr < depth
+            if rec[0] == 0:
+                old = np.where(ok, m[lanes, np.where(ok, addr, 0)], np.uint64(0))
+                planes = to_planes(old, 64)
+                for bit, slot in rec[3]:
+                    L[slot] = planes[bit]
+            else:
+                g = ((self.ref(L, rec[3]) >> lanes.astype(np.uint64)) & np.uint64(1)) == 1
+                data = self.lane_vals(L, rec[4])
+                w = g & on & ok
+                m[lanes[w], addr[w]] = data[w]
+
     def hash(self, L, k, h):
         for nbytes, width, refs in self.hashes[k]:
             planes = np.array([self.ref(L, int(r)) for r in refs], np.uint64)
@@ -128,21 +179,26 @@ class Program:
                     h = fnv_bytes(h, from_planes(planes[w0:w0 + 32]), 4)
         return h
 
-    def cycles(self, L, seed, sids, ncycles, act, h):
-        """The kernel's schedule: A0, commit; then per cycle the merged pass
-        M (outputs of cycle n, next state of cycle n + 1), commit, hash of
-        cycle n; B and the hash of the last cycle."""
+    def cycles(self, L, seed, sids, ncycles, act, h, mem=None):
+        """The kernel's schedule: A0, commit and memory ports; then per cycle
+        the merged pass M (outputs of cycle n, next state of cycle n + 1),
+        commit and memory ports, hash of cycle n; B and the hash of the last
+        cycle."""
         if ncycles == 0:
             return h
+        if mem is None:
+            mem = self.new_mems()
         self.inputs_to(L, seed, sids, 0)
         self.run_phase(L, 0)
         self.commit(L, 0, act)
+        self.ram_step(L, 0, act, mem)
         self.copy_inputs(L)
         if ncycles > 1:
             self.inputs_to(L, seed, sids, 1)
         for n in range(ncycles - 1):
             self.run_phase(L, 1)
             self.commit(L, 1, act)
+            self.ram_step(L, 1, act, mem)
             h = self.hash(L, 0, h)
             self.copy_inputs(L)
             if n + 2 < ncycles:

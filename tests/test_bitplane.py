@@ -36,11 +36,18 @@ def test_gates_are_folded_and_shared():
     assert n.OR(a, a ^ 1) == bp.T and n.AND(a, a ^ 1) == bp.F
 
 
-def test_refuses_memories_and_division():
-    with pytest.raises(bp.Unsupported):
+def test_refuses_unsupported_memories_and_division():
+    # an asynchronous read
+    with pytest.raises(bp.Unsupported, match="asynchronous"):
         comp = Compiled(elaborate("""module m(input clk, input [3:0] a, output [7:0] o);
-            reg [7:0] mem [0:15]; always @(posedge clk) mem[a] <= mem[a] + 8'd1;
+            reg [7:0] mem [0:15]; always @(posedge clk) mem[a] <= {4'd0, a};
             assign o = mem[a]; endmodule"""), Options(threads=1))
+        bp.compile_program(comp)
+    # read data feeding logic before the edge
+    with pytest.raises(bp.Unsupported, match="before the edge"):
+        comp = Compiled(elaborate("""module m(input clk, input [3:0] a, output reg [7:0] o);
+            reg [7:0] mem [0:15]; always @(posedge clk) begin mem[a] <= mem[a] + 8'd1; o <= mem[a]; end
+            endmodule"""), Options(threads=1))
         bp.compile_program(comp)
     with pytest.raises(bp.Unsupported):
         comp = Compiled(elaborate("""module m(input clk, input [7:0] a, input [7:0] b, output reg [7:0] o);
@@ -117,3 +124,36 @@ def test_write_back_chunks_restore_every_stored_value():
     man = comp.manifest()
     bad = [v["name"] for v in man["vars"] if v["name"] in want and got.get((v["cls"], v["loc"])) != want[v["name"]]]
     assert not bad, bad[:5]
+
+
+def test_netlist_with_rams_matches_oracle():
+    """Synchronous RAMs (registered read port, write port, out-of-range
+    addresses of a depth that is not a power of two) and a 70-bit input port
+    in two draws."""
+    src, table = generate(n_gates=400, n_ff=30, n_pi=70, n_po=20, depth=8, seed=31, n_rams=3,
+                          ram_depth=12, ram_width=9, name="nl")
+    comp, blob = _prog(src)
+    prog = Program(blob)
+    assert len(prog.mems) == 3 and all(len(p) == 6 for p in prog.ports)
+    got = prog.run(0x55AA, 70, 40)
+    want = oracle.netlist_run(table.to_bytes(), 0x55AA, 70, 40)
+    assert (got == want).all(), int((got != want).sum())
+
+
+
+
+@pytest.mark.parametrize("lane_major_min_depth", [64, 1 << 30])
+def test_memory_ports_match_host_build(lane_major_min_depth):
+    """Guarded writes under if / case, two write ports on one memory (the
+    later wins), reads of the address written at the same edge (old data),
+    8-, 6- and 64-bit memories, element-major and lane-major layouts, and
+    out-of-range addresses: the program against the per-lane host build."""
+    from rtlflow_amd.designs.rams import verilog as rams
+    from rtlflow_amd.hostsim import HostDesign
+    comp = Compiled(elaborate(rams()), Options(threads=2, lane_major_min_depth=lane_major_min_depth))
+    prog = bp.compile_program(comp)
+    kinds = [r[0] for r in prog.ports[1]]
+    assert kinds.count(0) == 4 and kinds.count(1) == 6
+    got = Program(bp.encode(prog)).run(0x1234, 70, 60)
+    want = HostDesign(comp).protocol(70, 0x1234, 60, mode=1)
+    assert (got == want).all(), int((got != want).sum())

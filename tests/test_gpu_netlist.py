@@ -66,3 +66,51 @@ def test_netlist_eval_api_matches_protocol(gpu):
             for s in range(n):
                 h[s] = oracle.fnv(int(po[s]).to_bytes(2, "little"), int(h[s]))
     assert (h == ref).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["net5s", "net5s_learned"])
+def test_net5s_rams_batched_queue_and_graph(gpu, name):
+    """Configuration 5's structure at one tenth of its gates and flip-flops:
+    eight 1,024 x 64-bit synchronous RAMs and 128 inputs drawn in two chunks.
+    Both partitions (instruction-count heuristic and learned) against the
+    oracle: the batched queue with lognormal budgets, and the graph form."""
+    from rtlflow_amd.designs import rv32i
+    if name not in registry.load_index():
+        pytest.skip(f"{name} is not built")
+    seed = registry.SEED_CFG[5]
+    tb = registry.table_bytes("net5s")
+    budgets = rv32i.budgets(seed, 96, median=300, lo=0, hi=1500)
+    ref = oracle.netlist_run(tb, seed, nstim=96, budgets=budgets)
+    cd = registry.get(name)
+    with cd.model(64) as m:
+        got = m.submit_queue(seed, budgets)
+    assert (got == ref).all(), f"{int((got != ref).sum())} of 96 stimuli differ"
+    ref = oracle.netlist_run(tb, seed, nstim=128, cycles=60)
+    with cd.model(128) as m:
+        m.begin(seed)
+        m.run_cycles(60, MODE_GRAPH)
+        got = m.hashes()
+    assert (got == ref).all()
+
+
+@pytest.mark.gpu
+def test_net5s_bitplane(gpu):
+    """The same design in the bit-plane form (its RAMs as memory ports): the
+    protocol and the batched workload against the oracle."""
+    from rtlflow_amd.designs import rv32i
+    from rtlflow_amd.model import MODE_BITPLANE
+    seed = registry.SEED_CFG[5]
+    tb = registry.table_bytes("net5s")
+    cd = registry.get("net5s")
+    assert cd.manifest["bitplane"]["supported"], cd.manifest["bitplane"]
+    ref = oracle.netlist_run(tb, seed, nstim=200, cycles=80)
+    with cd.model(200) as m:
+        m.begin(seed)
+        m.run_cycles(80, MODE_BITPLANE)
+        assert (m.hashes() == ref).all()
+    budgets = rv32i.budgets(seed, 150, median=300, lo=0, hi=1500)
+    ref = oracle.netlist_run(tb, seed, nstim=150, budgets=budgets)
+    with cd.model(128) as m:
+        got = m.submit_queue(seed, budgets, mode=MODE_BITPLANE)
+    assert (got == ref).all(), f"{int((got != ref).sum())} of 150 stimuli differ"

@@ -31,6 +31,21 @@ def test_netlist_host_build_all_forms(tiny, threads):
     assert (hd.queue(3, seed, budgets) == refb).all()
 
 
+def test_netlist_with_rams_and_wide_inputs_host_build():
+    """Synchronous RAMs (registered read port, out-of-range addresses) and a
+    70-bit input port drawn in two 64-bit chunks, in every schedule form."""
+    src, table = generate(n_gates=400, n_ff=24, n_pi=70, n_po=20, depth=8, seed=31, n_rams=3,
+                          ram_depth=12, ram_width=9)
+    tb = table.to_bytes()
+    seed = 0x52544C4600000005
+    ref = oracle.netlist_run(tb, seed, nstim=6, cycles=60)
+    hd = HostDesign(Compiled(elaborate(src), Options(threads=3)))
+    for mode in (0, 1, 2):
+        assert (hd.protocol(6, seed, 60, mode=mode, order_seed=2) == ref).all()
+    budgets = [60, 0, 13, 60, 1, 44]
+    assert (hd.queue(4, seed, budgets) == oracle.netlist_run(tb, seed, nstim=6, budgets=budgets)).all()
+
+
 def test_calls_form_matches_inline_form(tiny):
     """Large designs call shared non-inlined mtask bodies; forcing that form
     on a small design must give the same hashes."""

@@ -41,10 +41,16 @@ def test_stimulus_counter_definition():
 
 
 def _py_netlist(table, seed, s, cycles):
-    """Brute-force restatement of the netlist semantics in Python."""
+    """Brute-force restatement of the netlist semantics in Python: primary
+    input bits [64j, 64j + 64) from draw k = j; flip-flops and each RAM's
+    registered read port update at the rising edge from the clk = 0 values,
+    the read port taking the word from before the same edge's write."""
     n_first = table.n_pi + table.n_ff + table.n_ram_bits
     v = [0] * (n_first + table.n_gates)
     q = [0] * table.n_ff
+    n_rams, w = len(table.ram_we), table.ram_width
+    mem = [[0] * table.ram_depth for _ in range(n_rams)]
+    rd = [0] * n_rams
     h = 0xcbf29ce484222325
 
     def comb(pi):
@@ -52,6 +58,9 @@ def _py_netlist(table, seed, s, cycles):
             v[i] = (pi >> i) & 1
         for i in range(table.n_ff):
             v[table.n_pi + i] = q[i]
+        for r in range(n_rams):
+            for k in range(w):
+                v[table.n_pi + table.n_ff + r * w + k] = (rd[r] >> k) & 1
         for g in range(table.n_gates):
             a, b, c = (v[int(x)] for x in table.gate_in[g])
             t = int(table.gate_type[g])
@@ -59,9 +68,19 @@ def _py_netlist(table, seed, s, cycles):
                               MUX: b if a else c}[t]
 
     for cyc in range(cycles):
-        pi = _py_splitmix(seed ^ ((s * 0x9E3779B97F4A7C15) & M64) ^ (cyc << 8)) & ((1 << table.n_pi) - 1)
+        pi = 0
+        for j in range((table.n_pi + 63) // 64):
+            pi |= _py_splitmix(seed ^ ((s * 0x9E3779B97F4A7C15) & M64) ^ (cyc << 8) ^ j) << (64 * j)
+        pi &= (1 << table.n_pi) - 1
         comb(pi)
         d = [v[int(x)] for x in table.ff_d]
+        for r in range(n_rams):
+            addr = sum(v[int(x)] << k for k, x in enumerate(table.ram_addr[r]))
+            data = sum(v[int(x)] << k for k, x in enumerate(table.ram_wdata[r]))
+            old = mem[r][addr] if addr < table.ram_depth else 0
+            if v[int(table.ram_we[r])] and addr < table.ram_depth:
+                mem[r][addr] = data
+            rd[r] = old
         q[:] = d
         comb(pi)
         po = sum(v[int(x)] << k for k, x in enumerate(table.po))
@@ -76,6 +95,18 @@ def test_netlist_oracle_matches_bruteforce():
     got = oracle.netlist_run(table.to_bytes(), seed, nstim=5, cycles=7)
     for s in range(5):
         assert int(got[s]) == _py_netlist(table, seed, s, 7)
+
+
+def test_netlist_oracle_rams_and_wide_inputs_match_bruteforce():
+    """Two input draws (70 primary inputs) and three synchronous RAMs of a
+    depth that is not a power of two (out-of-range addresses read 0 and do
+    not write)."""
+    _, table = generate(n_gates=160, n_ff=10, n_pi=70, n_po=12, depth=5, seed=21, n_rams=3,
+                        ram_depth=6, ram_width=5)
+    seed = 0x52544C4600000005
+    got = oracle.netlist_run(table.to_bytes(), seed, nstim=4, cycles=40)
+    for s in range(4):
+        assert int(got[s]) == _py_netlist(table, seed, s, 40)
 
 
 def test_netlist_oracle_budgets_and_stim0():
