@@ -96,7 +96,11 @@ int rf_eval(rf_model* m);
  * mode RF_MODE_GRAPH replays the captured per-mtask graph once per cycle,
  * RF_MODE_FUSED runs the fused per-lane kernel, RF_MODE_BITPLANE runs the
  * bit-plane kernel (64 lanes per 64-bit word; RF_E_STATE if the design has
- * no bit-plane program). All three leave identical state and hashes. */
+ * no bit-plane program). All three leave identical state and hashes.
+ * rf_run_cycles queues its launches on the model's stream and returns; the
+ * next call that waits for the device (rf_sync, rf_hashes, rf_get_port, ...)
+ * reports their errors (RF_E_HIP, RF_E_CONVERGE) and adds their HIP-event
+ * times to the statistics. */
 int rf_begin(rf_model* m, uint64_t seed, uint32_t stim0);
 int rf_run_cycles(rf_model* m, uint32_t n, uint32_t mode);
 int rf_hashes(rf_model* m, uint32_t lane0, uint32_t count, uint64_t* out);

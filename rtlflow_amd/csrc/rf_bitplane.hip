@@ -320,20 +320,21 @@ rf_bp_kernel(rf_bp_params P) {
     // variables by one ballot each, wider ones by transposes
     {
         const uint32_t* st = prog + P.off_state;
-        // batches of 8 single-bit entries, round-robin over the wavefronts
-        for (uint32_t k0 = wave * 8u; k0 < P.n_state1; k0 += NW * 8u) {
-            const uint32_t cnt = min(8u, P.n_state1 - k0);
-            uint64_t v[8];
+        // batches of 16 single-bit entries (class C: one byte per lane),
+        // round-robin over the wavefronts; the batch's loads are issued back
+        // to back, so one memory latency covers the batch
+        constexpr uint32_t SB = 16;
+        for (uint32_t k0 = wave * SB; k0 < P.n_state1; k0 += NW * SB) {
+            const uint32_t cnt = min(SB, P.n_state1 - k0);
+            uint32_t v[SB];
 #pragma unroll
-            for (uint32_t j = 0; j < 8u; ++j) {
-                v[j] = 0;
-                if (j < cnt && inb) {
-                    const uint32_t* e = st + 6u * (k0 + j);
-                    v[j] = bp_load(a, e[0], (uint64_t)e[1] | ((uint64_t)e[2] << 32), lane);
-                }
+            for (uint32_t j = 0; j < SB; ++j) {
+                const uint32_t* e = st + 6u * (k0 + min(j, cnt - 1u));
+                const uint64_t loc = (uint64_t)e[1] | ((uint64_t)e[2] << 32);
+                v[j] = inb ? a.c[(size_t)loc * a.npad + lane] : 0u;
             }
 #pragma unroll
-            for (uint32_t j = 0; j < 8u; ++j) {
+            for (uint32_t j = 0; j < SB; ++j) {
                 if (j < cnt) {
                     const uint64_t pl = __ballot((int)(v[j] & 1u));
                     if (t == j) L[st[6u * (k0 + j) + 5]] = pl;
@@ -431,10 +432,16 @@ rf_bp_kernel(rf_bp_params P) {
         go += go & 1u;
         bp_phase<NT>(L, flev, reinterpret_cast<const uint2*>(fc + go), fnl, tid);
         const uint32_t* wp = fc + go + 2u * fng;
-        for (uint32_t k = wave; k < nwb1; k += NW) {
-            const uint32_t* e = wp + 6u * k;
-            const uint64_t v = (bp_ref(L, e[5]) >> t) & 1ull;
-            if (act) bp_store(a, e[0], (uint64_t)e[1] | ((uint64_t)e[2] << 32), lane, v);
+        // single-bit entries (class C), 8 per wavefront step
+        for (uint32_t k0 = wave * 8u; k0 < nwb1; k0 += NW * 8u) {
+#pragma unroll
+            for (uint32_t j = 0; j < 8u; ++j) {
+                if (k0 + j < nwb1) {
+                    const uint32_t* e = wp + 6u * (k0 + j);
+                    const uint64_t v = (bp_ref(L, e[5]) >> t) & 1ull;
+                    if (act) a.c[(size_t)((uint64_t)e[1] | ((uint64_t)e[2] << 32)) * a.npad + lane] = (uint8_t)v;
+                }
+            }
         }
         wp += 6u * nwb1;
         for (uint32_t k = nwb1; k < nwb; ++k) {
@@ -607,6 +614,7 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
             if (e + 5 > o + cw) BP_FAIL("bit-plane write-back entry truncated");
             const uint32_t cls = b[e], wds = b[e + 3], width = b[e + 4];
             if ((k < nwb1) != (width == 1)) BP_FAIL("bit-plane write-back list not ordered single-bit first");
+            if (width == 1 && cls != RF_CLS_C) BP_FAIL("bit-plane single-bit write-back entry not of class C");
             if (cls > 3 || width == 0 || e + 5 + width > o + cw ||
                 (width > 64 && (cls != RF_CLS_I || wds * 32u < width || wds * 32u >= width + 32u)))
                 BP_FAIL("bit-plane write-back entry malformed");
@@ -624,6 +632,7 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
     for (uint32_t k = 0; k < p->n_state; ++k) {
         const uint32_t* sl = b + p->off_state + 6 * k;
         if ((k < p->n_state1) != (sl[4] == 1)) BP_FAIL("bit-plane state list not ordered single-bit first");
+        if (sl[4] == 1 && sl[0] != RF_CLS_C) BP_FAIL("bit-plane single-bit state entry not of class C");
         if (sl[0] > 3 || sl[4] == 0 || sl[5] + sl[4] > p->nslots ||
             (sl[4] > 64 && (sl[0] != RF_CLS_I || sl[3] * 32u < sl[4] || sl[3] * 32u >= sl[4] + 32u)))
             BP_FAIL("bit-plane state entry out of range");

@@ -82,6 +82,15 @@ struct rf_model {
     rf_args base{};
     uint32_t* bp_prog = nullptr;   // device copy of the bit-plane program
     rf_bp_params bp{};
+    // launches of rf_run_cycles not yet waited for: their HIP events are
+    // folded into the statistics (and device flags checked) by the next call
+    // that waits for the device
+    struct pending {
+        hipEvent_t e0, e1;
+        uint32_t kind, count;
+    };
+    std::vector<pending> pend;
+    std::vector<hipEvent_t> evfree;
 };
 
 namespace {
@@ -227,6 +236,65 @@ int load_bitplane(rf_model* m) {
     return RF_OK;
 }
 
+enum { KIND_FUSED = 0, KIND_GRAPH = 1, KIND_BITPLANE = 2 };
+constexpr size_t kMaxPending = 64;
+
+int ev_take(rf_model* m, hipEvent_t* e) {
+    if (!m->evfree.empty()) {
+        *e = m->evfree.back();
+        m->evfree.pop_back();
+        return RF_OK;
+    }
+    RF_HIP(hipEventCreate(e));
+    return RF_OK;
+}
+
+// Open a timed region on the model's stream (*e0 recorded).
+int timed_begin(rf_model* m, hipEvent_t* e0) {
+    int rc = ev_take(m, e0);
+    if (rc) return rc;
+    RF_HIP(hipEventRecord(*e0, m->stream));
+    return RF_OK;
+}
+
+// Close it: ``count`` launches of ``kind`` are timed when next waited for.
+int timed_end(rf_model* m, hipEvent_t e0, uint32_t kind, uint32_t count) {
+    hipEvent_t e1;
+    int rc = ev_take(m, &e1);
+    if (rc) return rc;
+    RF_HIP(hipEventRecord(e1, m->stream));
+    m->pend.push_back({e0, e1, kind, count});
+    return RF_OK;
+}
+
+int check_flags(rf_model* m);
+
+// Wait for every queued launch, fold the HIP-event times into the
+// statistics and (``check``) report the run's device flags.
+int settle(rf_model* m, bool check = true) {
+    if (m->pend.empty()) return RF_OK;
+    hipError_t e = hipEventSynchronize(m->pend.back().e1);
+    for (const auto& q : m->pend) {
+        float ms = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, q.e0, q.e1);
+        if (q.kind == KIND_FUSED) {
+            m->stats.fused_launches += q.count;
+            m->stats.fused_ms += ms;
+        } else if (q.kind == KIND_GRAPH) {
+            m->stats.graph_launches += q.count;
+            m->stats.graph_ms += ms;
+        } else {
+            m->stats.bitplane_launches += q.count;
+            m->stats.bitplane_ms += ms;
+        }
+        m->evfree.push_back(q.e0);
+        m->evfree.push_back(q.e1);
+    }
+    m->pend.clear();
+    if (e != hipSuccess) return fail(RF_E_HIP, "queued launches: %s", hipGetErrorString(e));
+    return check ? check_flags(m) : RF_OK;
+}
+
 int do_init(rf_model* m) {
     if (m->did_init) return RF_OK;
     int rc = launch(m, m->d->k_init, args_for(m, RF_PHASE_EVAL));
@@ -364,6 +432,11 @@ int rf_create(const rf_design* d, uint32_t n_lanes, const rf_config* cfg, rf_mod
 int rf_destroy(rf_model* m) {
     if (!m) return RF_OK;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (const auto& q : m->pend) {
+        (void)hipEventDestroy(q.e0);
+        (void)hipEventDestroy(q.e1);
+    }
+    for (hipEvent_t e : m->evfree) (void)hipEventDestroy(e);
     if (m->cycle_exec) (void)hipGraphExecDestroy(m->cycle_exec);
     if (m->cycle_graph) (void)hipGraphDestroy(m->cycle_graph);
     if (m->eval_exec) (void)hipGraphExecDestroy(m->eval_exec);
@@ -385,6 +458,10 @@ int rf_destroy(rf_model* m) {
 
 static int port_check(rf_model* m, uint32_t port, uint32_t lane0, uint32_t count, const void* buf) {
     if (!m || !buf) return fail(RF_E_ARG, "null model or buffer");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
     if (port >= m->d->nports) return fail(RF_E_ARG, "port %u out of range (%u ports)", port, m->d->nports);
     if ((uint64_t)lane0 + count > m->nlanes)
         return fail(RF_E_ARG, "lanes [%u, %u) exceed %u lanes", lane0, lane0 + count, m->nlanes);
@@ -428,6 +505,10 @@ int rf_get_port(rf_model* m, uint32_t port, uint32_t lane0, uint32_t count, void
 static int mem_xfer(rf_model* m, uint32_t mi, uint32_t lane0, uint32_t count, uint32_t elem0,
                     uint32_t nelem, void* host, bool to_dev) {
     if (!m || !host) return fail(RF_E_ARG, "null model or buffer");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
     if (mi >= m->d->nmems) return fail(RF_E_ARG, "memory %u out of range", mi);
     const rf_mem& md = m->d->mems[mi];
     if ((uint64_t)lane0 + count > m->nlanes) return fail(RF_E_ARG, "lane range out of bounds");
@@ -480,6 +561,10 @@ int rf_mem_read(rf_model* m, uint32_t mem, uint32_t lane0, uint32_t count, uint3
 
 int rf_set_done(rf_model* m, uint32_t lane0, uint32_t count, const uint8_t* done) {
     if (!m || !done) return fail(RF_E_ARG, "null model or buffer");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
     if ((uint64_t)lane0 + count > m->nlanes) return fail(RF_E_ARG, "lane range out of bounds");
     RF_HIP(hipMemcpyAsync(m->done + lane0, done, count, hipMemcpyHostToDevice, m->stream));
     RF_HIP(hipStreamSynchronize(m->stream));
@@ -488,6 +573,10 @@ int rf_set_done(rf_model* m, uint32_t lane0, uint32_t count, const uint8_t* done
 
 int rf_eval(rf_model* m) {
     if (!m) return fail(RF_E_ARG, "null model");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
     int rc = do_init(m);
     if (rc) return rc;
     const rf_design* d = m->d;
@@ -520,6 +609,10 @@ int rf_eval(rf_model* m) {
 
 int rf_begin(rf_model* m, uint64_t seed, uint32_t stim0) {
     if (!m) return fail(RF_E_ARG, "null model");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
     std::vector<uint64_t> h(m->npad, 0xcbf29ce484222325ull);  // FNV-1a-64 offset basis
     std::vector<uint32_t> s(m->npad);
     for (uint64_t k = 0; k < m->npad; ++k) s[k] = stim0 + (uint32_t)k;
@@ -545,61 +638,51 @@ int rf_begin(rf_model* m, uint64_t seed, uint32_t stim0) {
 
 int rf_run_cycles(rf_model* m, uint32_t n, uint32_t mode) {
     if (!m) return fail(RF_E_ARG, "null model");
-    int rc = do_init(m);
+    if (mode > RF_MODE_BITPLANE) return fail(RF_E_ARG, "unknown run mode %u", mode);
+    if (mode == RF_MODE_BITPLANE && !m->bp_prog)
+        return fail(RF_E_STATE, "design %s has no bit-plane program", m->d->name);
+    int rc = m->pend.size() >= kMaxPending ? settle(m) : RF_OK;
     if (rc) return rc;
+    rc = do_init(m);
+    if (rc) return rc;
+    hipEvent_t e0;
     if (mode == RF_MODE_GRAPH) {
-        RF_HIP(hipEventRecord(m->ev0, m->stream));
+        if (!n) return RF_OK;
+        rc = timed_begin(m, &e0);
+        if (rc) return rc;
         for (uint32_t c = 0; c < n; ++c) RF_HIP(hipGraphLaunch(m->cycle_exec, m->stream));
-        RF_HIP(hipEventRecord(m->ev1, m->stream));
-        RF_HIP(hipEventSynchronize(m->ev1));
-        float ms = 0;
-        RF_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
-        m->stats.graph_launches += n;
-        m->stats.graph_ms += ms;
-    } else if (mode == RF_MODE_FUSED) {
-        uint32_t left = n;
-        while (left) {
-            const uint32_t chunk = std::min(left, m->cfg.fused_chunk);
+        return timed_end(m, e0, KIND_GRAPH, n);
+    }
+    uint32_t left = n;
+    while (left) {
+        const uint32_t chunk = std::min(left, m->cfg.fused_chunk);
+        rc = timed_begin(m, &e0);
+        if (rc) return rc;
+        if (mode == RF_MODE_FUSED) {
             rf_args a = args_for(m, RF_PHASE_EVAL);
             a.ncycles = chunk;
-            RF_HIP(hipEventRecord(m->ev0, m->stream));
             rc = launch(m, m->d->k_fused, a);
             if (rc) return rc;
-            RF_HIP(hipEventRecord(m->ev1, m->stream));
-            RF_HIP(hipEventSynchronize(m->ev1));
-            float ms = 0;
-            RF_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
-            m->stats.fused_launches++;
-            m->stats.fused_ms += ms;
-            left -= chunk;
-        }
-    } else if (mode == RF_MODE_BITPLANE) {
-        if (!m->bp_prog) return fail(RF_E_STATE, "design %s has no bit-plane program", m->d->name);
-        uint32_t left = n;
-        while (left) {
-            const uint32_t chunk = std::min(left, m->cfg.fused_chunk);
+        } else {
             rf_bp_params p = m->bp;
             rf_bp_rounds(m->d->bitplane, m->cfg.bitplane_threads, &p);
             p.a = args_for(m, RF_PHASE_EVAL);
             p.a.ncycles = chunk;
-            RF_HIP(hipEventRecord(m->ev0, m->stream));
             RF_HIP(rf_bp_launch(p, m->cfg.bitplane_threads, m->stream));
-            RF_HIP(hipEventRecord(m->ev1, m->stream));
-            RF_HIP(hipEventSynchronize(m->ev1));
-            float ms = 0;
-            RF_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
-            m->stats.bitplane_launches++;
-            m->stats.bitplane_ms += ms;
-            left -= chunk;
         }
-    } else {
-        return fail(RF_E_ARG, "unknown run mode %u", mode);
+        rc = timed_end(m, e0, mode == RF_MODE_FUSED ? KIND_FUSED : KIND_BITPLANE, 1);
+        if (rc) return rc;
+        left -= chunk;
     }
-    return check_flags(m);
+    return RF_OK;
 }
 
 int rf_hashes(rf_model* m, uint32_t lane0, uint32_t count, uint64_t* out) {
     if (!m || !out) return fail(RF_E_ARG, "null model or buffer");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
     if ((uint64_t)lane0 + count > m->nlanes) return fail(RF_E_ARG, "lane range out of bounds");
     RF_HIP(hipMemcpyAsync(out, m->hash + lane0, (size_t)count * 8, hipMemcpyDeviceToHost, m->stream));
     RF_HIP(hipStreamSynchronize(m->stream));
@@ -608,6 +691,10 @@ int rf_hashes(rf_model* m, uint32_t lane0, uint32_t count, uint64_t* out) {
 
 int rf_set_mem_images(rf_model* m, uint32_t mi, uint32_t nstim, uint32_t nelem, const void* src) {
     if (!m) return fail(RF_E_ARG, "null model");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
     if (mi >= m->d->nmems) return fail(RF_E_ARG, "memory %u out of range", mi);
     const rf_mem& md = m->d->mems[mi];
     if (nelem > md.depth) return fail(RF_E_ARG, "image larger than the memory");
@@ -668,6 +755,10 @@ static int queue_setup(rf_model* m, const uint32_t* budgets, uint32_t nstim, con
 int rf_submit_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
                     const uint32_t* order, uint64_t* out_hashes) {
     if (!m || !budgets || !out_hashes) return fail(RF_E_ARG, "null argument");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
     if (!nstim) return RF_OK;
     std::vector<uint32_t> ord;
     int qrc = queue_setup(m, budgets, nstim, order, ord);
@@ -702,6 +793,10 @@ int rf_submit_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_
 int rf_submit_queue_bitplane(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
                              const uint32_t* order, uint64_t* out_hashes) {
     if (!m || !budgets || !out_hashes) return fail(RF_E_ARG, "null argument");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
     if (!m->bp_prog) return fail(RF_E_STATE, "design %s has no bit-plane program", m->d->name);
     if (m->base.images)
         return fail(RF_E_STATE, "memory images are loaded by the per-lane batched form only");
@@ -766,6 +861,10 @@ int rf_submit_queue_bitplane(rf_model* m, uint64_t seed, const uint32_t* budgets
 
 int rf_lane_state(rf_model* m, uint32_t lane, void* dst, size_t* bytes) {
     if (!m || !bytes) return fail(RF_E_ARG, "null argument");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
     if (lane >= m->nlanes) return fail(RF_E_ARG, "lane %u out of range", lane);
     size_t total = 0;
     for (int c = 0; c < 4; ++c) total += m->d->slots[c] * kClassBytes[c];
@@ -797,6 +896,10 @@ int rf_lane_state(rf_model* m, uint32_t lane, void* dst, size_t* bytes) {
 
 int rf_profile_mtasks(rf_model* m, uint32_t reps, float* ms_out) {
     if (!m || !ms_out || !reps) return fail(RF_E_ARG, "bad argument");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
     int rc = do_init(m);
     if (rc) return rc;
     // every lane requests a pass, so kernels that skip settled lanes do work
@@ -826,12 +929,20 @@ int rf_profile_mtasks(rf_model* m, uint32_t reps, float* ms_out) {
 
 int rf_sync(rf_model* m) {
     if (!m) return fail(RF_E_ARG, "null model");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
     RF_HIP(hipStreamSynchronize(m->stream));
     return RF_OK;
 }
 
 int rf_flags(rf_model* m, uint32_t* out, uint32_t n) {
     if (!m || !out) return fail(RF_E_ARG, "null argument");
+    {
+        const int src = settle(m, false);
+        if (src) return src;
+    }
     if (n > RF_FLAG_WORDS) n = RF_FLAG_WORDS;
     RF_HIP(hipMemcpyAsync(out, m->flags, n * 4, hipMemcpyDeviceToHost, m->stream));
     RF_HIP(hipStreamSynchronize(m->stream));
@@ -840,12 +951,20 @@ int rf_flags(rf_model* m, uint32_t* out, uint32_t n) {
 
 int rf_get_stats(rf_model* m, rf_stats* out) {
     if (!m || !out) return fail(RF_E_ARG, "null argument");
+    {
+        const int src = settle(m, false);
+        if (src) return src;
+    }
     *out = m->stats;
     return RF_OK;
 }
 
 int rf_reset_stats(rf_model* m) {
     if (!m) return fail(RF_E_ARG, "null model");
+    {
+        const int src = settle(m, false);
+        if (src) return src;
+    }
     m->stats = rf_stats{};
     return RF_OK;
 }
