@@ -196,6 +196,16 @@ def main():
         bytes_per_launch = b_alg * lanes
         kernel = "cycle graph"
     achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
+    # HBM bytes per launch: rocprofv3 FETCH_SIZE / WRITE_SIZE of the same
+    # launch shape, scaled by a plain copy kernel (profiles/); null for other
+    # shapes
+    traffic = None
+    tp = ROOT / "profiles" / f"{args.design}_bitplane_traffic.json"
+    if main_form == "bitplane" and tp.exists():
+        t = json.loads(tp.read_text())
+        if (t["lanes"], t["cycles_per_launch"], t["bitplane_threads"]) == \
+                (lanes, min(cps, args.chunk), args.bp_threads or 512):
+            traffic = t["read_bytes"] + t["write_bytes"]
     ops = man["ops_per_cycle"]
     valu = stim_cycles * ops / forms[main_form]["seconds"]
 
@@ -233,7 +243,7 @@ def main():
         "learned_partition": learned,
         "bit_exact_sample": all(v["bit_exact_sample"] for v in forms.values()),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
                      "alg_bytes_per_stimulus_cycle": b_alg, "avg_launch_ms": per_launch_ms},
         "valu": {"scalar_equivalent_ops": valu, "peak_ops": VALU_PEAK, "frac": valu / VALU_PEAK,
                  "ops_per_stimulus_cycle": ops},
