@@ -413,7 +413,11 @@ def bench_net5s(args):
     if cd.manifest.get("bitplane", {}).get("supported"):
         # the bit-plane form: stimuli dealt to 64-lane groups longest budget
         # first, the RAMs as memory ports
-        nt = args.bp_threads or 1024
+        nt = args.bp_threads
+        if not nt:
+            from rtlflow_amd.autotune import choice
+            t = choice("net5s")
+            nt = t["knobs"]["bp_threads"] if t and t.get("form") == "bitplane" else 1024
         log(f"[bench] net5s bitplane: {nstim} stimuli on {lanes} lanes, {nt} threads per group")
         with cd.model(lanes, bitplane_threads=nt) as m:
             m.submit_queue(seed, budgets[:64], mode=MODE_BITPLANE)    # code load

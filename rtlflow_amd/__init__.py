@@ -41,8 +41,11 @@ class CompiledDesign:
 
 
 def compile_design(src: str, top: Optional[str] = None, params: Optional[Dict[str, int]] = None,
-                   name: Optional[str] = None, jobs: int = 8, **opts) -> CompiledDesign:
-    """Verilog text -> built design library (the rtlflow compiler + make)."""
+                   name: Optional[str] = None, jobs: int = 8, force: bool = False,
+                   **opts) -> CompiledDesign:
+    """Verilog text -> built design library (the rtlflow compiler + make).
+    A library whose generated source is unchanged is reused unless
+    ``force``."""
     from .build import build_design
     from .compiler import Compiled, Options, emit_source
     from .verilog import elaborate
@@ -52,7 +55,7 @@ def compile_design(src: str, top: Optional[str] = None, params: Optional[Dict[st
     text = emit_source(comp)
     man = comp.manifest()
     nm = name or design.name
-    lib = build_design(nm, {f"V{nm}.hip": text}, man, jobs=jobs)
+    lib = build_design(nm, {f"V{nm}.hip": text}, man, jobs=jobs, force=force)
     return CompiledDesign(lib, man, comp)
 
 

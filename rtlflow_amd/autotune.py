@@ -41,6 +41,7 @@ WORKLOADS = {
     "alu": {"kind": "cycles", "lanes": 1024, "cycles": 2000, "cfg": 1},
     "systolic": {"kind": "cycles", "lanes": 8192, "cycles": 500, "cfg": 4},
     "rv32i": {"kind": "queue", "nstim": 4096, "median": 2000, "cfg": 3},
+    "net5s": {"kind": "cycles", "lanes": 4096, "cycles": 200, "cfg": 5},
 }
 
 COMPILE_SPACE = {

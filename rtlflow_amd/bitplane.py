@@ -662,7 +662,9 @@ def build_phase(net: Net, roots: Sequence[Lit], fixed: Dict[int, int], nfixed: i
     by_level: List[List[int]] = [[] for _ in range(nlev + 1)]
     for x in order:
         by_level[level[x]].append(x)
-    free: List[int] = []
+    # free slots by bank residue: a run of BANK_GROUP gates takes
+    # destinations of distinct residues (conflict-free writes)
+    free: List[List[int]] = [[] for _ in range(BANK_MOD)]
     top = nfixed
     slot: Dict[int, int] = dict(fixed)
     release: Dict[int, List[int]] = {}
@@ -671,26 +673,94 @@ def build_phase(net: Net, roots: Sequence[Lit], fixed: Dict[int, int], nfixed: i
     peak = nfixed
     ph = Phase()
     ph.inplace = done_pairs
+
+    def take(avoid: set) -> int:
+        nonlocal top
+        best = -1
+        for r in range(BANK_MOD):
+            if r not in avoid and free[r] and (best < 0 or len(free[r]) > len(free[best])):
+                best = r
+        if best >= 0:
+            return free[best].pop()
+        while top % BANK_MOD in avoid:
+            free[top % BANK_MOD].append(top)
+            top += 1
+        top += 1
+        return top - 1
+
     for L in range(1, nlev + 1):
-        gates = []
+        pending = []
         for x in by_level[L]:
-            if x in direct:
-                s = direct[x]
-            elif free:
-                s = free.pop()
-            else:
-                s = top
-                top += 1
-            slot[x] = s
-            gates.append(_gate(net, x, slot))
+            slot[x] = 0            # destination chosen below
+            pending.append((x, _gate(net, x, slot)))
+        ordered = bank_order(pending)
+        gates = []
+        for r0 in range(0, len(ordered), BANK_GROUP):
+            avoid: set = set()
+            for x, g in ordered[r0:r0 + BANK_GROUP]:
+                if x in direct:
+                    d = direct[x]
+                else:
+                    d = take(avoid)
+                avoid.add(d % BANK_MOD)
+                slot[x] = d
+                gates.append((d,) + g[1:])
         peak = max(peak, top)
-        # sort by operation shape so neighbouring lanes take the same path
-        gates.sort(key=lambda g: (g[1], g[2]))
         ph.levels.append(gates)
         for y in release.get(L, []):
             if y in slot and y not in fixed and y not in direct:
-                free.append(slot[y])
+                free[slot[y] % BANK_MOD].append(slot[y])
     return ph, slot, peak
+
+
+# LDS bank model for the gate order of a level: 8-byte slots, slot i in the
+# bank pair (2i, 2i + 1), BANK_GROUP lanes served together, so a run of
+# BANK_GROUP gates is conflict-free when its slots differ modulo BANK_MOD
+BANK_GROUP = 16
+BANK_MOD = 16
+
+
+def bank_order(pending: List[Tuple[int, Tuple[int, int, int, int, int, int]]]
+               ) -> List[Tuple[int, Tuple[int, int, int, int, int, int]]]:
+    """Order one level's (node, gate) pairs so that each run of BANK_GROUP
+    consecutive gates (the lanes one LDS access serves) spreads its operand
+    reads (s, x, y) over the banks: a run takes a gate only while, per
+    operand, the slot's residue is held by fewer than ``cap`` other slots of
+    the run (equal slots are a broadcast and free); ``cap`` starts at 1 and
+    rises only when no remaining gate fits. Destinations are chosen
+    afterwards, distinct per run. Greedy, level by level."""
+    rest = sorted(pending, key=lambda p: (p[1][1], p[1][2]))
+    out = []
+    while rest:
+        used: List[Dict[int, set]] = [{}, {}, {}]
+        run: List[int] = []
+        taken = set()
+        cap = 1
+        while len(run) < min(BANK_GROUP, len(rest)):
+            hit = -1
+            for k, (_, g) in enumerate(rest):
+                if k in taken:
+                    continue
+                ok = True
+                for jj, o in enumerate((g[1], g[2], g[4])):
+                    held = used[jj].get(o % BANK_MOD)
+                    if held and o not in held and len(held) >= cap:
+                        ok = False
+                        break
+                if ok:
+                    hit = k
+                    break
+            if hit < 0:
+                cap += 1
+                continue
+            g = rest[hit][1]
+            for jj, o in enumerate((g[1], g[2], g[4])):
+                used[jj].setdefault(o % BANK_MOD, set()).add(o)
+            run.append(hit)
+            taken.add(hit)
+        out += [rest[k] for k in sorted(run)]
+        rest = [p for k, p in enumerate(rest) if k not in taken]
+    return out
 
 
 def _lit_slot(lit: Lit, slot: Dict[int, int]) -> Tuple[int, int]:

@@ -3,9 +3,11 @@
 // clk = 1 settle, output hash) for 64 stimuli per 64-bit word.
 //
 // One workgroup (128 to 1,024 threads) per group of 64 lanes. Wavefronts 0
-// and 1 do the per-lane work (stimulus generation, transposes into planes,
-// output hashing, state load and write-back); every thread evaluates gates,
-// one level of the program per barrier, with all values in LDS:
+// and 1 do the per-cycle lane work (stimulus generation, transposes into
+// planes, output hashing); synchronous memory ports go to the other
+// wavefronts; all wavefronts share the state load and write-back at launch
+// start and end. Every thread evaluates gates, one level of the program per
+// barrier, with all values in LDS:
 //
 //   dst = s ? (x ^ ix) : (y ^ iy)        (a bit-select per 32-bit half)
 //
@@ -264,10 +266,11 @@ __device__ __forceinline__ void bp_rounds(uint64_t* L, const uint2 (&d)[MAXR], c
 }
 
 // Workgroup of NT threads (NT >= 128) per 64 lanes. Lane t of the group is
-// thread t of wavefront 0 (stimulus generation, state load, write-back) and
-// thread 64 + t of wavefront 1 (output hash). After phase B of cycle n,
-// wavefront 1 hashes cycle n while wavefront 0 generates cycle n + 1.
-// MAXR > 0: every thread keeps its gate records of phases A and B (at most
+// thread t of every wavefront: wavefront 0 generates stimuli, wavefront 1
+// hashes outputs, wavefront (2 + i) mod NW serves memory i. After program M
+// of cycle n, wavefront 1 hashes cycle n while wavefront 0 generates cycle
+// n + 2's inputs.
+// MAXR > 0: every thread keeps its gate records of program M (at most
 // MAXR rounds) in registers for the whole launch, so the cycle loop issues
 // no global-memory loads; MAXR = 0: records are streamed from memory.
 // Up to 24 register-resident rounds the kernel fits in 80 registers, which

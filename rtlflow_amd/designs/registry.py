@@ -91,7 +91,7 @@ def _file_text(path: Path) -> dict:
 COSTTABLES = Path(__file__).resolve().parent.parent / "costtables"
 
 
-def build_entry(name: str, jobs: int = 8) -> dict:
+def build_entry(name: str, jobs: int = 8, force: bool = False) -> dict:
     from .. import compile_design
     e = ENTRIES[name]
     t0 = time.time()
@@ -106,7 +106,7 @@ def build_entry(name: str, jobs: int = 8) -> dict:
         # partition with the learned per-vertex costs instead of instruction counts
         from ..costmodel import TableCost
         opts["cost_fn"] = TableCost(json.loads((COSTTABLES / f"{table}.json").read_text()))
-    cd = compile_design(spec["verilog"], top=spec["top"], name=name, jobs=jobs, **opts)
+    cd = compile_design(spec["verilog"], top=spec["top"], name=name, jobs=jobs, force=force, **opts)
     rec = {"name": name, "lib": str(cd.lib_path.relative_to(BUILD)), "opts": e.opts,
            "build_s": round(time.time() - t0, 1)}
     from ..build import build_testbench
@@ -119,16 +119,32 @@ def build_entry(name: str, jobs: int = 8) -> dict:
     return rec
 
 
-def build_all(names=None, jobs: int = 8, log=print) -> dict:
+def build_all(names=None, jobs: int = 8, log=print, force: bool = False, workers: int = 1) -> dict:
+    """Build registered designs (all by default). ``force`` recompiles
+    libraries whose generated source is unchanged; ``workers`` > 1 builds
+    that many designs at once in separate processes, the slowest (by their
+    last build time) first."""
     index = load_index()
-    for name in (names or list(ENTRIES)):
-        rec = build_entry(name, jobs=jobs)
+    names = list(names or ENTRIES)
+
+    def done(name, rec):
         index[name] = rec
         if log:
             log(f"built {name}: {rec['lib']} ({rec['build_s']} s)")
         INDEX.parent.mkdir(parents=True, exist_ok=True)
         INDEX.write_text(json.dumps(index, indent=1))
         _prune(name, Path(rec["lib"]).parent.name)
+
+    if workers <= 1:
+        for name in names:
+            done(name, build_entry(name, jobs=jobs, force=force))
+        return index
+    from concurrent.futures import ProcessPoolExecutor, as_completed
+    names.sort(key=lambda n: -index.get(n, {}).get("build_s", 0.0))
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        futs = {ex.submit(build_entry, n, 1, force): n for n in names}
+        for f in as_completed(futs):
+            done(futs[f], f.result())
     return index
 
 
