@@ -12,7 +12,7 @@ import hashlib
 import json
 from typing import Callable, Dict, List, Optional, Set
 
-from .emit_hip import CTY, Gen, cty, edge_cond, is_wide, lit
+from .emit_hip import ARR, CTY, Gen, cty, edge_cond, is_wide, lit
 from .instrcount import process_cost
 from .ir import CLASS_BYTES, Design, Ref, Var, words_of
 from .layout import Layout, class_of
@@ -849,10 +849,57 @@ def _fused_locals(comp: Compiled) -> Set[Var]:
     return loc
 
 
+# memories of at most this many bytes per lane are held in LDS by the
+# whole-cycle kernels (fused and persistent), within this many bytes of LDS
+# per workgroup
+LDS_MEM_MAX_BYTES = 256
+LDS_MEM_BUDGET = 32768
+
+
+def lds_memories(comp: Compiled) -> Dict[Var, str]:
+    """Small memories (a register file) that the whole-cycle kernels keep in
+    LDS: one column per thread, so a lane's reads and writes cost an LDS
+    access instead of a global one. Not in the host build or the calls form
+    (whose mtask bodies also serve the per-mtask kernels)."""
+    if comp.host or comp.calls:
+        return {}
+    out: Dict[Var, str] = {}
+    total = 0
+    for m in comp.design.vars:
+        if not (m.is_mem and comp.layout.has(m)):
+            continue
+        b = m.depth * m.words * CLASS_BYTES[class_of(m)]
+        if b <= LDS_MEM_MAX_BYTES and total + b * comp.opts.block <= LDS_MEM_BUDGET:
+            out[m] = f"lm{m.id}"
+            total += b * comp.opts.block
+    return out
+
+
+def _lds_decl_lines(comp: Compiled, lds: Dict[Var, str]) -> List[str]:
+    return [f"    __shared__ {CTY[class_of(m)]} {nm}[{m.depth * m.words}][{comp.opts.block}];"
+            for m, nm in lds.items()]
+
+
+def _lds_copy_lines(comp: Compiled, lds: Dict[Var, str], to_lds: bool) -> List[str]:
+    """Copy this lane's small memories between their storage and LDS."""
+    lay = comp.layout
+    out = []
+    for m, nm in lds.items():
+        n = m.depth * m.words
+        if m in lay.lane_major:
+            src = f"mb{m.id}[k]"
+        else:
+            src = f"{ARR[class_of(m)]}[((size_t){lay.loc[m]} + k) * NP]"
+        a, b = (f"{nm}[k][threadIdx.x]", src) if to_lds else (src, f"{nm}[k][threadIdx.x]")
+        out.append(f"    for (uint32_t k = 0; k < {n}u; ++k) {a} = {b};")
+    return out
+
+
 def emit_fused(comp: Compiled) -> str:
     """Protocol kernel: ``a.ncycles`` cycles per lane with all logic inlined."""
     d = comp.design
     gen = Gen(comp)
+    gen.lds = lds_memories(comp)
     gen.top_lines = []
     gen.local = _fused_locals(comp)
     plan_packing(comp, gen, comp.stored)
@@ -887,8 +934,11 @@ def emit_fused(comp: Compiled) -> str:
         if v in comp.stored:
             gen.store(v)
     lines = [kernel_header(comp, f"rf_k_fused")]
+    lines += _lds_decl_lines(comp, gen.lds)
     _prologue(comp, gen, lines, check_done=True)
-    lines += gen.top_lines + gen.lines + ["}"]
+    lines += _lds_copy_lines(comp, gen.lds, to_lds=True)
+    lines += gen.top_lines + gen.lines
+    lines += _lds_copy_lines(comp, gen.lds, to_lds=False) + ["}"]
     return "\n".join(lines)
 
 

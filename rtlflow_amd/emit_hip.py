@@ -83,6 +83,9 @@ class Gen:
         # same clocked group); None = every NBA target
         self.nb_needed: Optional[Set[Var]] = None
         self.write_through = False
+        # small memories held in LDS by whole-cycle kernels: var -> array name
+        # (element word k of this thread at name[k][threadIdx.x])
+        self.lds: Dict[Var, str] = {}
 
     # ------------------------------------------------------------ output
     def emit(self, s: str):
@@ -176,6 +179,8 @@ class Gen:
 
     # --------------------------------------------------------- memories
     def mem_elem(self, m: Var, idx: str, word: int = 0) -> str:
+        if m in self.lds:
+            return f"{self.lds[m]}[(size_t)({idx}) * {m.words} + {word}][threadIdx.x]"
         lay = self.comp.layout
         if m in lay.lane_major:
             c = class_of(m)

@@ -15,17 +15,20 @@ rule, which bounds the tail by the longest single stimulus).
 """
 from __future__ import annotations
 
-from .compiler import (_eval_loop_body, _fused_locals, _hash_code, _initial_body, _prologue,
-                       _stim_code, atomic, kernel_header, plan_packing)
+from .compiler import (_eval_loop_body, _fused_locals, _hash_code, _initial_body, _lds_copy_lines,
+                       _lds_decl_lines, _prologue, _stim_code, atomic, kernel_header, lds_memories,
+                       plan_packing)
 from .emit_hip import Gen, is_wide
 from .ir import CLASS_BYTES
 from .layout import class_of
 
 
-def image_load_lines(comp, ind: str) -> list:
+def image_load_lines(comp, ind: str, lds=None) -> list:
     """Copy the stimulus's memory images (rf_set_mem_images) into the lane.
     a.images is a table of {image base pointer, (nstim << 32) | nelem} per
-    memory of the design's memory table."""
+    memory of the design's memory table. ``lds``: memories the kernel holds
+    in LDS (var -> array name), loaded there instead."""
+    lds = lds or {}
     d = comp.design
     lay = comp.layout
     mems = [v for v in d.vars if v.is_mem and lay.has(v)]
@@ -43,7 +46,9 @@ def image_load_lines(comp, ind: str) -> list:
         out.append(f"{ind}      const uint32_t ne = (uint32_t)im[{2 * j + 1}];")
         out.append(f"{ind}      if (src && sid < (uint32_t)(im[{2 * j + 1}] >> 32)) {{")
         out.append(f"{ind}        src += (size_t)sid * ne * {n}u;")
-        if m in lay.lane_major:
+        if m in lds:
+            out.append(f"{ind}        for (uint32_t k = 0; k < ne * {n}u; ++k) {lds[m]}[k][threadIdx.x] = src[k];")
+        elif m in lay.lane_major:
             row = m.depth * (CLASS_BYTES[m.cls] if m.cls != "W" else 4 * m.words)
             out.append(f"{ind}        {T}* dst = ({T}*)(a.mem + (size_t){lay.lane_major[m]} * a.npad"
                        f" + (size_t)lane * {row}u);")
@@ -59,6 +64,7 @@ def image_load_lines(comp, ind: str) -> list:
 def emit_persistent(comp) -> str:
     d = comp.design
     gen = Gen(comp)
+    gen.lds = lds_memories(comp)
     gen.top_lines = []
     gen.local = _fused_locals(comp)
     plan_packing(comp, gen, comp.stored)
@@ -80,14 +86,16 @@ def emit_persistent(comp) -> str:
         gen.emit(f"{r} = 0u;")
     gen.avail = set(gen.local)
     for m in [v for v in d.vars if v.is_mem and lay.has(v)]:
-        if m in lay.lane_major:
+        if m in gen.lds:
+            gen.emit(f"for (uint32_t k = 0; k < {m.depth * m.words}u; ++k) {gen.lds[m]}[k][threadIdx.x] = 0;")
+        elif m in lay.lane_major:
             row = m.depth * (CLASS_BYTES[m.cls] if m.cls != "W" else 4 * m.words)
             gen.emit(f"{{ uint64_t* p = (uint64_t*)mb{m.id}; for (uint32_t k = 0; k < {row // 8}u; ++k) p[k] = 0ull; }}")
         else:
             c = class_of(m)
             arr = {"C": "sC", "S": "sS", "I": "sI", "Q": "sQ"}[c]
             gen.emit(f"for (uint32_t k = 0; k < {m.slots}u; ++k) {arr}[((size_t){lay.loc[m]} + k) * NP] = 0;")
-    for ln in image_load_lines(comp, ""):
+    for ln in image_load_lines(comp, "", gen.lds):
         gen.emit(ln)
     _initial_body(comp, gen)
     gen.emit("uint64_t h = RF_FNV_BASIS;")
@@ -111,6 +119,13 @@ def emit_persistent(comp) -> str:
     gen.ind -= 1
     gen.emit("}")
     lines = [kernel_header(comp, "rf_k_persistent")]
+    lines += _lds_decl_lines(comp, gen.lds)
     _prologue(comp, gen, lines, check_done=False)
-    lines += gen.top_lines + gen.lines + ["}"]
+    lines += gen.top_lines + gen.lines
+    # the last stimulus's memories back to their storage (lanes that ran one)
+    if gen.lds:
+        lines.append("    if (lane < a.nstim) {")
+        lines += ["    " + ln for ln in _lds_copy_lines(comp, gen.lds, to_lds=False)]
+        lines.append("    }")
+    lines.append("}")
     return "\n".join(lines)
