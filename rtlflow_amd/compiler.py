@@ -880,6 +880,68 @@ def _lds_decl_lines(comp: Compiled, lds: Dict[Var, str]) -> List[str]:
             for m, nm in lds.items()]
 
 
+# whole-cycle kernels keep at most this many words of variables in
+# registers; beyond it, variables move to LDS (within LDS_STATE_BUDGET bytes
+# per workgroup, small memories included)
+REG_WORDS = 160
+LDS_STATE_BUDGET = 144 * 1024
+
+
+def place_lds_state(comp: Compiled, gen: Gen) -> List[str]:
+    """Move register variables of a whole-cycle kernel to LDS when they
+    exceed REG_WORDS words (a design whose state would otherwise spill to
+    scratch memory, which is global memory). Wide variables move first, then
+    narrow ones in declaration order, while the workgroup's LDS budget
+    lasts. Narrow variables live in one array per class, [slot][thread]
+    (a wavefront's accesses to one variable are consecutive); wide ones in a
+    thread-major array with an odd row length (a wavefront's accesses to one
+    word fall in distinct banks). Returns the declarations."""
+    if comp.host:
+        return []
+    B = comp.opts.block
+    regs = [v for v in gen.local if v not in gen.packed]
+    words = sum(v.words if is_wide(v.width) else (2 if v.width > 32 else 1) for v in regs)
+    if words <= REG_WORDS:
+        return []
+    used = sum(m.depth * m.words * CLASS_BYTES[class_of(m)] for m in gen.lds) * B
+    order = sorted((v for v in regs if is_wide(v.width)), key=lambda v: (-v.words, v.id)) + \
+        sorted((v for v in regs if not is_wide(v.width)), key=lambda v: v.id)
+    nslot = {"C": 0, "S": 0, "I": 0, "Q": 0}
+    wide_off = 0
+    moved_narrow, moved_wide = [], []
+    for v in order:
+        if words <= REG_WORDS:
+            break
+        if is_wide(v.width):
+            cost = v.words * 4 * B
+            if used + cost > LDS_STATE_BUDGET:
+                continue
+            moved_wide.append((v, wide_off))
+            wide_off += v.words
+            words -= v.words
+        else:
+            c = class_of(v)
+            cost = CLASS_BYTES[c] * B
+            if used + cost > LDS_STATE_BUDGET:
+                continue
+            moved_narrow.append((v, c, nslot[c]))
+            nslot[c] += 1
+            words -= 2 if v.width > 32 else 1
+        used += cost
+    lines = []
+    for c in ("C", "S", "I", "Q"):
+        if nslot[c]:
+            lines.append(f"    __shared__ {CTY[c]} ls{c}[{nslot[c]}][{B}];")
+    for v, c, k in moved_narrow:
+        gen.lds_vars[v] = f"ls{c}[{k}][threadIdx.x]"
+    if moved_wide:
+        row = wide_off | 1
+        lines.append(f"    __shared__ uint32_t lsW[{B}][{row}];")
+        for v, off in moved_wide:
+            gen.lds_vars[v] = f"&lsW[threadIdx.x][{off}]"
+    return lines
+
+
 def _lds_copy_lines(comp: Compiled, lds: Dict[Var, str], to_lds: bool) -> List[str]:
     """Copy this lane's small memories between their storage and LDS."""
     lay = comp.layout
@@ -903,6 +965,7 @@ def emit_fused(comp: Compiled) -> str:
     gen.top_lines = []
     gen.local = _fused_locals(comp)
     plan_packing(comp, gen, comp.stored)
+    lds_state = place_lds_state(comp, gen)
     # registers that carry values across evals are loaded once
     for v in sorted(gen.local, key=lambda x: x.id):
         if v in comp.persistent:
@@ -934,7 +997,7 @@ def emit_fused(comp: Compiled) -> str:
         if v in comp.stored:
             gen.store(v)
     lines = [kernel_header(comp, f"rf_k_fused")]
-    lines += _lds_decl_lines(comp, gen.lds)
+    lines += _lds_decl_lines(comp, gen.lds) + lds_state
     _prologue(comp, gen, lines, check_done=True)
     lines += _lds_copy_lines(comp, gen.lds, to_lds=True)
     lines += gen.top_lines + gen.lines

@@ -86,6 +86,10 @@ class Gen:
         # small memories held in LDS by whole-cycle kernels: var -> array name
         # (element word k of this thread at name[k][threadIdx.x])
         self.lds: Dict[Var, str] = {}
+        # register variables kept in LDS instead (the state of designs too
+        # large for the register file): narrow var -> lvalue of this
+        # thread's cell; wide var -> initialiser of its word pointer
+        self.lds_vars: Dict[Var, str] = {}
 
     # ------------------------------------------------------------ output
     def emit(self, s: str):
@@ -104,6 +108,9 @@ class Gen:
         p = self.packed.get(v)
         if p is not None:
             return f"rf_bget({p[0]}, {p[1]})"
+        cell = self.lds_vars.get(v)
+        if cell is not None:
+            return f"(({cty(v.width)}){cell})"
         return self.name(v)
 
     def set(self, v: Var, expr: str) -> str:
@@ -111,6 +118,9 @@ class Gen:
         p = self.packed.get(v)
         if p is not None:
             return f"{p[0]} = rf_bset({p[0]}, {p[1]}, {expr});"
+        cell = self.lds_vars.get(v)
+        if cell is not None:
+            return f"{cell} = ({CTY[class_of(v)]})({expr});"
         return f"{self.name(v)} = {expr};"
 
     def soa(self, v: Var, word: int = 0) -> str:
@@ -127,6 +137,13 @@ class Gen:
             if p[0] not in self.pack_regs:
                 self.pack_regs.add(p[0])
                 self.emit_top(f"uint32_t {p[0]} = 0u;")
+            return
+        cell = self.lds_vars.get(v)
+        if cell is not None:
+            if is_wide(v.width):
+                self.emit_top(f"uint32_t* const {self.name(v)} = {cell};")
+            else:
+                self.emit_top(f"{cell} = 0;")
             return
         if is_wide(v.width):
             self.emit_top(f"uint32_t {self.name(v)}[{v.words}];")

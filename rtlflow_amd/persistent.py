@@ -17,7 +17,7 @@ from __future__ import annotations
 
 from .compiler import (_eval_loop_body, _fused_locals, _hash_code, _initial_body, _lds_copy_lines,
                        _lds_decl_lines, _prologue, _stim_code, atomic, kernel_header, lds_memories,
-                       plan_packing)
+                       place_lds_state, plan_packing)
 from .emit_hip import Gen, is_wide
 from .ir import CLASS_BYTES
 from .layout import class_of
@@ -68,6 +68,7 @@ def emit_persistent(comp) -> str:
     gen.top_lines = []
     gen.local = _fused_locals(comp)
     plan_packing(comp, gen, comp.stored)
+    lds_state = place_lds_state(comp, gen)
     for v in sorted(gen.local, key=lambda x: x.id):
         gen.declare(v)
     lay = comp.layout
@@ -81,7 +82,7 @@ def emit_persistent(comp) -> str:
         if is_wide(v.width):
             gen.emit(f"rfw_zero({gen.name(v)}, {v.width});")
         elif v not in gen.packed:
-            gen.emit(f"{gen.name(v)} = 0;")
+            gen.emit(gen.set(v, "0"))
     for r in sorted(gen.pack_regs):
         gen.emit(f"{r} = 0u;")
     gen.avail = set(gen.local)
@@ -119,7 +120,7 @@ def emit_persistent(comp) -> str:
     gen.ind -= 1
     gen.emit("}")
     lines = [kernel_header(comp, "rf_k_persistent")]
-    lines += _lds_decl_lines(comp, gen.lds)
+    lines += _lds_decl_lines(comp, gen.lds) + lds_state
     _prologue(comp, gen, lines, check_done=False)
     lines += gen.top_lines + gen.lines
     # the last stimulus's memories back to their storage (lanes that ran one)
