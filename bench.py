@@ -430,6 +430,31 @@ def bench_net5s(args):
                              "checked_stimuli": nchk, "lanes": lanes, "threads": nt,
                              "stimulus_cycles": total}
         log(f"[bench] bitplane {dt:.3f}s {total / dt:.4g} stim-cycles/s oracle_match={ok}")
+    # the learned partition in the form its cost model predicts (the
+    # per-mtask graph form: the sum of an eval's mtask kernel times)
+    lc = 200
+    lres = {}
+    for tag, name in (("heuristic", "net5s"), ("learned", "net5s_learned")):
+        c = registry.get(name)
+        log(f"[bench] net5s partition={tag} graph form, {nstim} x {lc} cycles, {c.manifest['mtasks']} mtasks")
+        with c.model(nstim) as m:
+            m.begin(seed)
+            m.run_cycles(0, MODE_GRAPH)
+            m.sync()
+            t0 = time.perf_counter()
+            m.run_cycles(lc, MODE_GRAPH)
+            m.sync()
+            dtl = time.perf_counter() - t0
+            lres[tag] = (nstim * lc / dtl, m.hashes(), c.manifest["mtasks"])
+    lref = oracle.netlist_run(tb, seed, 8, cycles=lc, threads=8)
+    lsame = bool((lres["heuristic"][1] == lres["learned"][1]).all())
+    lexact = lsame and bool((lres["learned"][1][:8] == lref).all())
+    lsp = lres["learned"][0] / lres["heuristic"][0]
+    learned = {"form": "graph", "cycles": lc, "heuristic": lres["heuristic"][0],
+               "learned": lres["learned"][0], "heuristic_mtasks": lres["heuristic"][2],
+               "learned_mtasks": lres["learned"][2], "speedup": lsp, "bit_exact": lexact,
+               "kept": bool(lexact and lsp > 1.0)}
+    log(f"[bench] net5s learned/heuristic graph throughput {lsp:.3f} (bit-exact {lexact})")
     med = int(np.median(budgets))
     cd = registry.get("net5s")
     with cd.model(nstim) as m:
@@ -465,6 +490,7 @@ def bench_net5s(args):
                    "design": "net5s", "form": main_form},
         "forms": forms, "bit_exact_sample": all(v["bit_exact_sample"] for v in forms.values()),
         "learned_vs_heuristic": forms["batched_learned"]["value"] / forms["batched"]["value"],
+        "learned_partition": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
