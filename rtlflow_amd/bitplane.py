@@ -1016,13 +1016,17 @@ def _final_bits(comp, lw: Lowerer, v: Var) -> Optional[List[Lit]]:
 # =====================================================================
 
 BP_MAGIC = 0x50424652   # "RFBP"
-BP_VERSION = 7
+BP_VERSION = 8
+# gate records hold LDS byte offsets (slot * 8) when every slot fits 16 bits
+BYTE_FORM_MAX_SLOTS = 8192
 HDR_WORDS = 18
 FCHUNK_HDR = 6
 
 
-def gate_word(g: Tuple[int, int, int, int, int, int]) -> int:
+def gate_word(g: Tuple[int, int, int, int, int, int], byte_form: bool = False) -> int:
     dst, s, xs, ix, ys, iy = g
+    if byte_form:
+        return (8 * dst) | ((8 * s) << 16) | ((8 * xs | ix) << 32) | ((8 * ys | iy) << 48)
     return dst | (s << 16) | (xs << 32) | (ix << 47) | (ys << 48) | (iy << 63)
 
 
@@ -1036,7 +1040,8 @@ def _wb_words(entries) -> List[int]:
 def encode(prog: Program) -> List[int]:
     """u32 words of the program blob.
 
-    header (18 words): magic, version, nslots, n_in, n_state, n_commit[A0],
+    header (18 words): magic, version | byte_form << 16, nslots, n_in,
+    n_state, n_commit[A0],
     n_hash, n_fchunks, nlev[A0, M, B], ngates[A0, M, B], n_state1, total
     words, n_commit[M], n_mem. The first n_state1 state entries are
     single-bit.
@@ -1050,9 +1055,12 @@ def encode(prog: Program) -> List[int]:
                   write into their state slots directly)
       commit M    n_commit[M] x the same after program M
       phase A0, M, B: level offsets (nlev + 1 gate indices, padded to even),
-                  then the gates as u64 (lo word first):
+                  then the gates as u64 (lo word first), evaluating
+                  dst = s ? (x ^ ix) : (y ^ iy) on 64-lane planes; slot form
                   dst | s << 16 | x << 32 | ix << 47 | y << 48 | iy << 63,
-                  dst = s ? (x ^ ix) : (y ^ iy) on 64-lane planes
+                  or with byte_form (at most BYTE_FORM_MAX_SLOTS slots) the
+                  LDS byte offsets 8 * slot, flags in bit 0:
+                  8dst | 8s << 16 | (8x | ix) << 32 | (8y | iy) << 48
       hash M, hash B: n_hash x {nbytes, width, width slot refs} each
       memories    n_mem x {design memory index, cls, layout (1: lane-major),
                   loc_lo, loc_hi, row bytes, depth, width}
@@ -1070,6 +1078,7 @@ def encode(prog: Program) -> List[int]:
     A slot ref is a slot index with the inversion in bit 31.
     """
     out: List[int] = [0] * HDR_WORDS
+    byte_form = prog.nslots <= BYTE_FORM_MAX_SLOTS
 
     def pad(x):
         if len(x) & 1:
@@ -1082,7 +1091,7 @@ def encode(prog: Program) -> List[int]:
         pad(w)
         for lev in ph.levels:
             for g in lev:
-                gw = gate_word(g)
+                gw = gate_word(g, byte_form)
                 w += [gw & 0xffffffff, gw >> 32]
         return w
 
@@ -1125,7 +1134,8 @@ def encode(prog: Program) -> List[int]:
         out += [len(ph.levels), sum(len(l) for l in ph.levels), len(entries), n1,
                 FCHUNK_HDR + len(body), 0] + body
     pad(out)
-    hdr = [BP_MAGIC, BP_VERSION, prog.nslots, len(prog.in_chunks), len(prog.state_load),
+    hdr = [BP_MAGIC, BP_VERSION | (int(byte_form) << 16), prog.nslots, len(prog.in_chunks),
+           len(prog.state_load),
            len(prog.commits[0]), len(prog.hashes[0]), len(prog.fchunks)]
     hdr += [len(ph.levels) for ph in prog.phases]
     hdr += [sum(len(l) for l in ph.levels) for ph in prog.phases]
@@ -1139,7 +1149,7 @@ def sections(blob: Sequence[int]) -> Dict[str, object]:
     """Word offsets of every section of an encoded program (the runtime
     computes the same offsets from the header)."""
     h = blob[:HDR_WORDS]
-    assert h[0] == BP_MAGIC and h[1] == BP_VERSION
+    assert h[0] == BP_MAGIC and (h[1] & 0xffff) == BP_VERSION
     n_in, n_state, n_commit_a, n_hash, n_f = h[3:8]
     n_commit = (n_commit_a, h[16])
     nlev = h[8:11]

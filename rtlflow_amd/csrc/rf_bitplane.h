@@ -12,7 +12,7 @@
 
 #define RF_BP_DEFAULT_THREADS 512
 #define RF_BP_MAGIC 0x50424652u
-#define RF_BP_VERSION 7u
+#define RF_BP_VERSION 8u
 #define RF_BP_MEM_WORDS 8u
 #define RF_BP_FCHUNK_HDR 6u
 #define RF_BP_HDR_WORDS 18u
@@ -37,6 +37,7 @@ struct rf_bp_params {
     uint32_t n_in, n_state, n_commit[2], n_hash, n_f;
     uint32_t n_state1;          // leading single-bit entries of the state list
     uint32_t n_mem;             // memories with ports (8-word entries at off_mem)
+    uint32_t byte_addr;         // gate records in the byte form (fields are LDS byte offsets)
     uint32_t off_in, off_state, off_commit[2], off_hash[2], off_f;
     uint32_t off_mem, off_ports[2];   // memory table; port lists after A0 and M
     uint32_t nlev[3], off_lev[3], off_gates[3];

@@ -18,6 +18,8 @@
 
 namespace {
 
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+
 __device__ __forceinline__ uint64_t bp_ref(const uint64_t* L, uint32_t r) {
     const uint64_t v = L[r & 0x7fffffffu];
     return (r >> 31) ? ~v : v;
@@ -61,8 +63,34 @@ __device__ __forceinline__ uint64_t bp_gather(const uint64_t* L, const uint32_t*
     return bp_transpose64(row, t);
 }
 
-// Gate record: x = dst | s << 16; y = xs | ix << 15 | ys << 16 | iy << 31.
+// Gate record, slot form: x = dst | s << 16; y = xs | ix << 15 | ys << 16 |
+// iy << 31. Byte form (BA, programs of at most 8,192 slots): every field is
+// a slot's LDS byte offset (slot * 8), with the inversion flags in bit 0 of
+// the x and y fields: x = dst8 | s8 << 16; y = (xs8 | ix) | (ys8 | iy) << 16.
+// The byte form needs no shift per operand address.
+template <bool BA>
 __device__ __forceinline__ void bp_gate(uint64_t* L, uint2 w) {
+    if constexpr (BA) {
+        // the planes are the kernel's only LDS allocation (no static
+        // __shared__ variable), so they start at LDS address 0 and a
+        // record's byte offsets are LDS addresses
+        (void)L;
+        const uint32_t dst = w.x & 0xffffu;
+        const uint32_t s = w.x >> 16;
+        const uint32_t xs = w.y & 0xfff8u;
+        const uint32_t ys = (w.y >> 16) & 0xfff8u;
+        const uint32_t ix = (uint32_t)((int32_t)(w.y << 31) >> 31);   // 0 or ~0
+        const uint32_t iy = (uint32_t)((int32_t)(w.y << 15) >> 31);
+        const uint64_t S = *reinterpret_cast<const lds_u64*>((size_t)s);
+        const uint64_t X = *reinterpret_cast<const lds_u64*>((size_t)xs);
+        const uint64_t Y = *reinterpret_cast<const lds_u64*>((size_t)ys);
+        const uint32_t xl = (uint32_t)X ^ ix, xh = (uint32_t)(X >> 32) ^ ix;
+        const uint32_t yl = (uint32_t)Y ^ iy, yh = (uint32_t)(Y >> 32) ^ iy;
+        const uint32_t sl = (uint32_t)S, sh = (uint32_t)(S >> 32);
+        *reinterpret_cast<lds_u64*>((size_t)dst) =
+            ((uint64_t)((sh & xh) | (~sh & yh)) << 32) | (uint64_t)((sl & xl) | (~sl & yl));
+        return;
+    }
     const uint32_t dst = w.x & 0xffffu;
     const uint32_t s = w.x >> 16;
     const uint32_t xs = w.y & 0x7fffu;
@@ -82,7 +110,7 @@ __device__ __forceinline__ void bp_gate(uint64_t* L, uint2 w) {
 // each level. Each thread's first descriptor of the next level is loaded
 // before the current level is evaluated, so its global-memory latency
 // overlaps the current level and the barrier.
-template <int NT>
+template <int NT, bool BA>
 __device__ __forceinline__ void bp_phase(uint64_t* L, const uint32_t* lev, const uint2* gates,
                                          uint32_t nlev, uint32_t tid) {
     if (nlev == 0) return;
@@ -98,8 +126,8 @@ __device__ __forceinline__ void bp_phase(uint64_t* L, const uint32_t* lev, const
             if (lo + tid < hi) nxt = gates[lo + tid];
         }
         if (g0 < g1) {
-            bp_gate(L, cur);
-            for (uint32_t g = g0 + NT; g < g1; g += NT) bp_gate(L, gates[g]);
+            bp_gate<BA>(L, cur);
+            for (uint32_t g = g0 + NT; g < g1; g += NT) bp_gate<BA>(L, gates[g]);
         }
         __syncthreads();
     }
@@ -248,7 +276,7 @@ __device__ __forceinline__ void bp_commit(uint64_t* L, const uint32_t* c, uint32
 
 // Rounds [i0, i1) of the register-resident gate records, a barrier where
 // the round table says a level ends.
-template <int MAXR>
+template <int MAXR, bool BA>
 __device__ __forceinline__ void bp_rounds(uint64_t* L, const uint2 (&d)[MAXR], const rf_bp_params& P,
                                           int i0, int i1) {
 #pragma unroll
@@ -259,7 +287,7 @@ __device__ __forceinline__ void bp_rounds(uint64_t* L, const uint2 (&d)[MAXR], c
             // per record instead of two)
             uint2 w = d[i];
             __asm__ volatile("" : "+v"(w.x), "+v"(w.y));
-            if (w.x != 0xffffffffu) bp_gate(L, w);
+            if (w.x != 0xffffffffu) bp_gate<BA>(L, w);
             if ((P.bar >> i) & 1ull) __syncthreads();
         }
     }
@@ -280,7 +308,7 @@ __device__ __forceinline__ void bp_rounds(uint64_t* L, const uint2 (&d)[MAXR], c
 template <int NT, int MAXR>
 constexpr int bp_waves_per_eu() { return NT >= 1024 ? 4 : (MAXR > 0 && MAXR <= 24 ? 6 : 1); }
 
-template <int NT, int MAXR>
+template <int NT, int MAXR, bool BA>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu<NT, MAXR>())))
 rf_bp_kernel(rf_bp_params P) {
     extern __shared__ uint64_t L[];
@@ -304,18 +332,17 @@ rf_bp_kernel(rf_bp_params P) {
     const uint32_t wave = tid >> 6;
     // cycles this lane advances: a.ncycles, or with per-stimulus budgets
     // (a.budget) what is left of its budget; the group runs as many cycles
-    // as its longest lane needs
+    // as its longest lane needs (every wavefront reduces the same 64 lanes,
+    // so no LDS word is needed and the planes start at LDS offset 0)
     const bool bud = a.budget != nullptr;
     uint32_t lim = a.ncycles;
-    __shared__ uint32_t s_ncycles;
-    if (bud && (w0 || w1)) {
-        const uint32_t b = act ? a.budget[sid] : 0u;
-        lim = b > cyc ? min(a.ncycles, b - cyc) : 0u;
-        if (w0) {
-            uint32_t mx = lim;
-            for (uint32_t k = 32; k; k >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, (int)k, 64));
-            if (t == 0) s_ncycles = mx;
-        }
+    uint32_t ngroup = a.ncycles;
+    if (bud) {
+        const uint32_t c0 = inb ? a.cyc[lane] : 0u;
+        const uint32_t b = act ? a.budget[a.stim[lane]] : 0u;
+        lim = b > c0 ? min(a.ncycles, b - c0) : 0u;
+        ngroup = lim;
+        for (uint32_t k = 32; k; k >>= 1) ngroup = max(ngroup, (uint32_t)__shfl_xor((int)ngroup, (int)k, 64));
     }
     if (tid == 0) L[0] = 0;
     // state planes from the lane-minor arrays, spread over all wavefronts
@@ -381,12 +408,12 @@ rf_bp_kernel(rf_bp_params P) {
             }
         }
     }
-    const uint32_t N = bud ? s_ncycles : a.ncycles;
+    const uint32_t N = ngroup;
     if (N) {
         // cycle 0's next state
         // (the commit writes state slots only; the lane work touches input
         // and output-copy slots only: one barrier covers both)
-        bp_phase<NT>(L, lev[RF_BP_A0], gates[RF_BP_A0], P.nlev[RF_BP_A0], tid);
+        bp_phase<NT, BA>(L, lev[RF_BP_A0], gates[RF_BP_A0], P.nlev[RF_BP_A0], tid);
         bp_commit<NT>(L, prog + P.off_commit[0], P.n_commit[0], tid);
         if (P.n_mem) bp_mem_ports(L, P, 0, wave, NW, t, lane, inb, act);
         if (w0) {
@@ -397,9 +424,9 @@ rf_bp_kernel(rf_bp_params P) {
         // cycle n's outputs and cycle n + 1's next state in one pass
         for (uint32_t n = 0; n + 1 < N; ++n) {
             if constexpr (MAXR > 0)
-                bp_rounds<MR>(L, d, P, 0, (int)P.nr);
+                bp_rounds<MR, BA>(L, d, P, 0, (int)P.nr);
             else
-                bp_phase<NT>(L, lev[RF_BP_M], gates[RF_BP_M], P.nlev[RF_BP_M], tid);
+                bp_phase<NT, BA>(L, lev[RF_BP_M], gates[RF_BP_M], P.nlev[RF_BP_M], tid);
             bp_commit<NT>(L, prog + P.off_commit[1], P.n_commit[1], tid);
             if (P.n_mem) bp_mem_ports(L, P, 1, wave, NW, t, lane, inb, act);
             if (w1) {
@@ -412,7 +439,7 @@ rf_bp_kernel(rf_bp_params P) {
             __syncthreads();
         }
         // the last cycle's outputs
-        bp_phase<NT>(L, lev[RF_BP_B], gates[RF_BP_B], P.nlev[RF_BP_B], tid);
+        bp_phase<NT, BA>(L, lev[RF_BP_B], gates[RF_BP_B], P.nlev[RF_BP_B], tid);
         if (w1) {
             const uint64_t hn = bp_hash(L, P, 1, h, t);
             if (act && N - 1 < lim) h = hn;
@@ -433,7 +460,7 @@ rf_bp_kernel(rf_bp_params P) {
         const uint32_t* flev = fc + RF_BP_FCHUNK_HDR;
         uint32_t go = RF_BP_FCHUNK_HDR + fnl + 1;
         go += go & 1u;
-        bp_phase<NT>(L, flev, reinterpret_cast<const uint2*>(fc + go), fnl, tid);
+        bp_phase<NT, BA>(L, flev, reinterpret_cast<const uint2*>(fc + go), fnl, tid);
         const uint32_t* wp = fc + go + 2u * fng;
         // single-bit entries (class C), 8 per wavefront step
         for (uint32_t k0 = wave * 8u; k0 < nwb1; k0 += NW * 8u) {
@@ -478,16 +505,24 @@ rf_bp_kernel(rf_bp_params P) {
 }  // namespace
 
 static bool bp_check_phase(const uint32_t* lv, uint32_t nlev, const uint32_t* g, uint32_t ngates,
-                           uint32_t nslots) {
+                           uint32_t nslots, bool byte_addr) {
     if (lv[0] != 0 || lv[nlev] != ngates) return false;
     for (uint32_t l = 0; l < nlev; ++l)
         if (lv[l + 1] < lv[l]) return false;
     for (uint32_t i = 0; i < ngates; ++i) {
         const uint64_t w = (uint64_t)g[2 * i] | ((uint64_t)g[2 * i + 1] << 32);
-        const uint32_t f[4] = {(uint32_t)(w & 0xffff), (uint32_t)((w >> 16) & 0xffff),
-                               (uint32_t)((w >> 32) & 0x7fff), (uint32_t)((w >> 48) & 0x7fff)};
-        for (uint32_t x : f)
-            if (x >= nslots) return false;
+        if (byte_addr) {
+            // dst and s: multiples of 8; x and y: multiples of 8 plus the flag bit
+            const uint32_t f[4] = {(uint32_t)(w & 0xffff), (uint32_t)((w >> 16) & 0xffff),
+                                   (uint32_t)((w >> 32) & 0xfffe), (uint32_t)((w >> 48) & 0xfffe)};
+            for (uint32_t x : f)
+                if ((x & 7u) || (x >> 3) >= nslots) return false;
+        } else {
+            const uint32_t f[4] = {(uint32_t)(w & 0xffff), (uint32_t)((w >> 16) & 0xffff),
+                                   (uint32_t)((w >> 32) & 0x7fff), (uint32_t)((w >> 48) & 0x7fff)};
+            for (uint32_t x : f)
+                if (x >= nslots) return false;
+        }
     }
     return true;
 }
@@ -505,8 +540,10 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
         *why = (msg);  \
         return -1;     \
     } while (0)
-    if (words < RF_BP_HDR_WORDS || b[0] != RF_BP_MAGIC || b[1] != RF_BP_VERSION)
+    if (words < RF_BP_HDR_WORDS || b[0] != RF_BP_MAGIC || (b[1] & 0xffffu) != RF_BP_VERSION ||
+        (b[1] >> 16) > 1u)
         BP_FAIL("bad bit-plane program header");
+    p->byte_addr = b[1] >> 16;
     p->nslots = b[2];
     p->n_in = b[3];
     p->n_state = b[4];
@@ -524,6 +561,7 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
     if (p->n_state1 > p->n_state) BP_FAIL("bit-plane single-bit count exceeds the state list");
     if (p->nslots == 0 || p->nslots > 0x7fffu || (uint64_t)p->nslots * 8u > 160u * 1024u - 64u)
         BP_FAIL("bit-plane program needs more LDS than a workgroup has");
+    if (p->byte_addr && p->nslots > 8192u) BP_FAIL("bit-plane byte-form records need at most 8,192 slots");
     auto ev = [](uint64_t x) { return x + (x & 1); };
     uint64_t o = ev(RF_BP_HDR_WORDS);
     p->off_in = (uint32_t)o;
@@ -546,7 +584,8 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
     }
     if (o > words) BP_FAIL("bit-plane sections exceed the program");
     for (int k = 0; k < 3; ++k)
-        if (!bp_check_phase(b + p->off_lev[k], p->nlev[k], b + p->off_gates[k], ngates[k], p->nslots))
+        if (!bp_check_phase(b + p->off_lev[k], p->nlev[k], b + p->off_gates[k], ngates[k], p->nslots,
+                            p->byte_addr))
             BP_FAIL("bit-plane program A0/M/B malformed");
     for (int which = 0; which < 2; ++which) {
         o = ev(o);
@@ -611,7 +650,7 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
         g = ev(g);
         uint64_t e = g + 2ull * ng;
         if (e > o + cw) BP_FAIL("bit-plane write-back chunk malformed");
-        if (!bp_check_phase(b + o + RF_BP_FCHUNK_HDR, nl, b + g, ng, p->nslots))
+        if (!bp_check_phase(b + o + RF_BP_FCHUNK_HDR, nl, b + g, ng, p->nslots, p->byte_addr))
             BP_FAIL("bit-plane write-back phase malformed");
         for (uint32_t k = 0; k < nwb; ++k) {
             if (e + 5 > o + cw) BP_FAIL("bit-plane write-back entry truncated");
@@ -671,19 +710,24 @@ void rf_bp_rounds(const uint32_t* b, uint32_t threads, rf_bp_params* p) {
     p->nr = r;
 }
 
-template <int NT, int MAXR>
-static hipError_t bp_launch_k(const rf_bp_params& p, hipStream_t stream) {
+template <int NT, int MAXR, bool BA>
+static hipError_t bp_launch_ba(const rf_bp_params& p, hipStream_t stream) {
     const size_t shmem = (size_t)p.nslots * 8u;
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rf_bp_kernel<NT, MAXR>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rf_bp_kernel<NT, MAXR, BA>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     const uint32_t grid = (p.a.nlanes + 63u) / 64u;
-    hipLaunchKernelGGL((rf_bp_kernel<NT, MAXR>), dim3(grid), dim3(NT), shmem, stream, p);
+    hipLaunchKernelGGL((rf_bp_kernel<NT, MAXR, BA>), dim3(grid), dim3(NT), shmem, stream, p);
     return hipGetLastError();
+}
+
+template <int NT, int MAXR>
+static hipError_t bp_launch_k(const rf_bp_params& p, hipStream_t stream) {
+    return p.byte_addr ? bp_launch_ba<NT, MAXR, true>(p, stream) : bp_launch_ba<NT, MAXR, false>(p, stream);
 }
 
 template <int NT>

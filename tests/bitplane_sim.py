@@ -57,6 +57,7 @@ class Program:
         self.nslots, self.n_in, self.n_state, self.n_commit, self.n_hash = h[2:7]
         self.nlev = h[8:11]
         self.ngates = h[11:14]
+        self.byte_form = bool(h[1] >> 16)
         b = self.b
         s = self.sec
         self.inputs = b[s["in"]:s["in"] + 4 * self.n_in].reshape(-1, 4).astype(np.int64)
@@ -114,6 +115,10 @@ class Program:
         g = b[gates_at:gates_at + 2 * ngates].reshape(-1, 2)
         w = g[:, 0] | (g[:, 1] << np.uint64(32))
         f = lambda sh, bits: ((w >> np.uint64(sh)) & np.uint64((1 << bits) - 1)).astype(np.int64)
+        if self.byte_form:
+            # LDS byte offsets, inversion flags in bit 0 of the x and y fields
+            return offs, (f(0, 16) >> 3, f(16, 16) >> 3, f(32, 16) >> 3, f(32, 1), f(48, 16) >> 3,
+                          f(48, 1))
         return offs, (f(0, 16), f(16, 16), f(32, 15), f(47, 1), f(48, 15), f(63, 1))
 
     def run_phase(self, L, k):

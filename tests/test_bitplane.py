@@ -17,10 +17,16 @@ def _prog(src, threads=2):
     return comp, bp.encode(bp.compile_program(comp))
 
 
+@pytest.mark.parametrize("byte_form", [True, False])
 @pytest.mark.parametrize("seed", [7, 8])
-def test_netlist_program_matches_oracle(seed):
+def test_netlist_program_matches_oracle(seed, byte_form, monkeypatch):
+    """Both gate record forms: LDS byte offsets (programs of at most 8,192
+    slots) and slot indices (larger programs, forced here)."""
+    if not byte_form:
+        monkeypatch.setattr(bp, "BYTE_FORM_MAX_SLOTS", 0)
     src, table = generate(n_gates=400, n_ff=40, n_pi=16, n_po=24, depth=10, seed=seed, name="nl")
     _, blob = _prog(src)
+    assert bool(blob[1] >> 16) == byte_form
     got = Program(blob).run(0x55AA, 70, 25)
     want = oracle.netlist_run(table.to_bytes(), 0x55AA, 70, 25)
     assert (got == want).all(), int((got != want).sum())
