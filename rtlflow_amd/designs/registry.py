@@ -248,6 +248,17 @@ def _rams():
 
 register("rams", _rams(), threads=2)
 
+# combinational cycles: one that settles, one that never does
+def _settle(which: str):
+    def make():
+        from . import settle
+        return {"verilog": getattr(settle, which.upper()), "top": which}
+    return make
+
+
+register("circ", _settle("circ"), threads=2)
+register("osc", _settle("osc"), threads=2)
+
 # random designs inside the bit-plane subset (no memories, no division)
 for _s in (1, 2, 3):
     register(f"bfuzz{_s}", _bfuzz(_s), threads=2)

@@ -1,0 +1,84 @@
+"""GPU: the settle loop (change detection on combinational cycles, the
+OR-reduced change flag, the on-device loop of the fused form and the
+host-driven loop of eval()) against a closed form, and the reporting of a
+loop that never settles."""
+import numpy as np
+import pytest
+
+import oracle
+from rtlflow_amd.designs import registry
+from rtlflow_amd.model import MODE_FUSED, MODE_GRAPH
+
+SEED = 0x5E77
+
+
+def _closed(nstim, cycles):
+    out = np.zeros(nstim, np.uint64)
+    for s in range(nstim):
+        h = 0xcbf29ce484222325
+        for c in range(cycles):
+            a = oracle.stim(SEED, s, c, 0) & 0xff
+            h = oracle.fnv(bytes([a & 0x0f, a]), h)
+        out[s] = h
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [MODE_GRAPH, MODE_FUSED])
+def test_settle_loop_protocol(gpu, mode):
+    cd = registry.get("circ")
+    n, cycles = 130, 25
+    want = _closed(n, cycles)
+    with cd.model(n) as m:
+        m.begin(SEED)
+        m.run_cycles(cycles, mode)
+        got = m.hashes()
+    assert (got == want).all()
+
+
+@pytest.mark.gpu
+def test_settle_loop_eval_api_and_queue(gpu):
+    """eval() runs the reference's host-driven settle loop (graph replays
+    until no lane requests a change)."""
+    cd = registry.get("circ")
+    n, cycles = 64, 6
+    with cd.model(n) as m:
+        h = np.full(n, 0xcbf29ce484222325, dtype=np.uint64)
+        for c in range(cycles):
+            a = np.array([oracle.stim(SEED, s, c, 0) & 0xff for s in range(n)], np.uint8)
+            m.set_port("a", a)
+            m.set_port("clk", np.zeros(n, np.uint8))
+            m.eval()
+            m.set_port("clk", np.ones(n, np.uint8))
+            m.eval()
+            o, ox = m.get_port("o"), m.get_port("ox")
+            assert (o == (a & 0x0f)).all() and (ox == a).all()
+            for s in range(n):
+                h[s] = oracle.fnv(bytes([int(o[s]), int(ox[s])]), int(h[s]))
+    assert (h == _closed(n, cycles)).all()
+    budgets = np.array([25, 0, 3, 17] * 10, np.uint32)
+    with cd.model(16) as m:
+        got = m.submit_queue(SEED, budgets)
+    for i, b in enumerate(budgets):
+        h = 0xcbf29ce484222325
+        for c in range(int(b)):
+            a = oracle.stim(SEED, i, c, 0) & 0xff
+            h = oracle.fnv(bytes([a & 0x0f, a]), h)
+        assert int(got[i]) == h, i
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [MODE_GRAPH, MODE_FUSED])
+def test_unsettled_loop_reports(gpu, mode):
+    """A loop that never settles: the run reports non-convergence at the
+    next call that waits for the device, in every form."""
+    cd = registry.get("osc")
+    with cd.model(64, converge_limit=20) as m:
+        m.begin(SEED)
+        m.run_cycles(3, mode)
+        with pytest.raises(RuntimeError, match="converge"):
+            m.hashes()
+    with cd.model(64, converge_limit=20) as m:
+        m.set_port("clk", np.zeros(64, np.uint8))
+        with pytest.raises(RuntimeError, match="converge"):
+            m.eval()

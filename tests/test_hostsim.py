@@ -111,3 +111,37 @@ def test_rv32i_host_build_and_iss():
             assert (hd.protocol(n, seed, 2500, mode=mode, order_seed=9, images={"imem": progs}) == ref).all()
         b = [2500, 7, 0, 1200]
         assert (hd.queue(2, seed, b, images={"imem": progs}) == oracle.rv32i_run(progs, seed, b)).all()
+
+
+def _circ_closed_form(seed, nstim, cycles, stim0=0):
+    """circ: after each clock, o = a & 0x0f and ox = a (the settled x)."""
+    out = []
+    for s in range(stim0, stim0 + nstim):
+        h = 0xcbf29ce484222325
+        for c in range(cycles):
+            a = oracle.stim(seed, s, c, 0) & 0xff
+            h = oracle.fnv(bytes([a & 0x0f, a]), h)
+        out.append(h)
+    return out
+
+
+def test_settle_loop_host_build():
+    """A combinational cycle through two always blocks (UNOPTFLAT) settles
+    by the change-detection loop in every schedule form and the batched
+    queue; the closed form pins the result. A cycle that never settles
+    reports non-convergence."""
+    import ir_interp
+    from rtlflow_amd.designs import settle
+    seed = 0x5E77
+    comp = Compiled(elaborate(settle.CIRC), Options(threads=2))
+    assert [v.name for v in comp.sch.change_vars] == ["y"]
+    want = _circ_closed_form(seed, 6, 20)
+    assert ir_interp.protocol(elaborate(settle.CIRC), seed, 6, 20) == want
+    hd = HostDesign(comp)
+    for mode in (0, 1, 2):
+        assert list(hd.protocol(6, seed, 20, mode=mode, order_seed=3)) == want
+    budgets = [20, 0, 7, 20, 1, 13]
+    got = list(hd.queue(3, seed, budgets))
+    assert got == [_circ_closed_form(seed, 1, b, stim0=i)[0] for i, b in enumerate(budgets)]
+    with pytest.raises(RuntimeError):
+        HostDesign(Compiled(elaborate(settle.OSC), Options(threads=2))).protocol(2, seed, 3, mode=1)
