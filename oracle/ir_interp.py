@@ -11,10 +11,13 @@ Semantics restated (two-state Verilog as Verilator computes it, with
   the operand's top bit as the sign (include/rf_verilated.h:1761-1817);
 * shifts by at least the width give 0 (or all sign bits for >>>), selects and
   memory reads out of range give 0, writes out of range are dropped, x / 0 = 0;
-* an eval: clocked blocks whose edge fired run against the values from before
-  the edge (non-blocking assignments committed after all of them), then the
-  combinational logic is iterated to a fixpoint (the settle loop; acyclic
-  logic reaches it after one ordered pass), then the clock latch is updated.
+* an eval: the combinational logic settles on the new inputs (so a clock
+  derived from them, such as a gated clock, takes its new value), clocked
+  blocks whose edge fired run against the values from before the edge
+  (non-blocking assignments committed after all of them), then the
+  combinational logic is iterated to a fixpoint again (the settle loop;
+  acyclic logic reaches it after one ordered pass), then every clock's
+  latch, derived clocks included, is updated.
 
 Op semantics here are written from the definitions above, not shared with the
 compiler's constant folder.
@@ -241,8 +244,10 @@ class Lane:
                     self.run(p.stmts, nba, True)
                     for t in nba:
                         self.write(*t)
+            self.comb_fixpoint()
             for c in self.d.clocks:
                 self.clklast[c] = self.v[c]
+        else:
             self.comb_fixpoint()
         fired = []
         for p in self.d.processes:

@@ -61,6 +61,9 @@ class Compiled:
             split_seq_processes(d)
         # synthetic clock-latch variables (__Vclklast__, src/V3Clock.cpp:64-92)
         self.clklast: Dict[Var, Var] = {}
+        # the clocks the protocol drives (primary inputs); derived clocks are
+        # combinational logic of them and get their own latch
+        self.in_clocks: List[Var] = [c for c in d.clocks if c.kind == "input"]
         for clk in d.clocks:
             v = Var(f"__Vclklast__{clk.name}", 1, False, "reg")
             v.id = len(d.vars)
@@ -466,8 +469,8 @@ def emit_bodies(comp: Compiled) -> List[str]:
     # clock value
     gen = Gen(comp)
     gen.top_lines = []
-    gen.local = set(d.clocks)
-    for clk in d.clocks:
+    gen.local = set(comp.in_clocks)
+    for clk in comp.in_clocks:
         gen.declare(clk)
         gen.emit(gen.set(clk, "v"))
         gen.store(clk)
@@ -662,7 +665,7 @@ def emit_eval_end(comp: Compiled) -> str:
         gen.emit("if (a.phase != RF_PHASE_EVAL) a.change[lane] = 1u;")
     gen.emit("if (a.phase == RF_PHASE_RISE) {")
     gen.ind += 1
-    for clk in d.clocks:
+    for clk in comp.in_clocks:
         gen.declare(clk)
         gen.emit(gen.set(clk, "1u"))
         gen.dirty.add(clk)
@@ -689,7 +692,7 @@ def emit_stim(comp: Compiled) -> str:
     gen.top_lines = []
     gen.local = set(d.inputs)
     _stim_code(comp, gen, "(uint64_t)a.stim[lane]", "(uint64_t)a.cyc[lane]")
-    for clk in d.clocks:
+    for clk in comp.in_clocks:
         gen.declare(clk)
         gen.emit(gen.set(clk, "0u"))
         gen.dirty.add(clk)
@@ -982,7 +985,7 @@ def emit_fused(comp: Compiled) -> str:
     gen.emit("#pragma unroll 1")
     gen.emit("for (uint32_t ph = 0u; ph < 2u; ++ph) {")
     gen.ind += 1
-    for clk in d.clocks:
+    for clk in comp.in_clocks:
         gen.emit(gen.set(clk, "ph"))
         gen.dirty.add(clk)
     _eval_loop_body(comp, gen)

@@ -258,6 +258,7 @@ def _settle(which: str):
 
 register("circ", _settle("circ"), threads=2)
 register("osc", _settle("osc"), threads=2)
+register("gated", _settle("gated"), threads=2)
 
 # random designs inside the bit-plane subset (no memories, no division)
 for _s in (1, 2, 3):

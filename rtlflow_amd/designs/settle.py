@@ -1,5 +1,6 @@
-"""Settle-loop coverage designs (Verilator's UNOPTFLAT case): combinational
-logic with a cycle through two always blocks.
+"""Settle-loop and clocking coverage designs.
+
+Combinational cycles through two always blocks (Verilator's UNOPTFLAT case):
 
 * ``circ`` converges: ``y`` takes only the upper nibble of ``x`` and ``x``
   only the lower nibble of ``y`` (always 0), so after the clk = 0 settle
@@ -29,3 +30,39 @@ module osc(input clk, input a, output o);
     assign o = q;
 endmodule
 """
+
+# A gated clock: ``gclk = clk & en`` (en changes with the other inputs while
+# clk is low), so ``g`` adds ``d`` exactly in the cycles whose ``en`` is set;
+# ``c`` counts every cycle and ``s`` shifts in ``d[0]``.
+GATED = r"""
+module gated(input clk, input en, input [3:0] d, output [7:0] cnt, output [3:0] sr, output [7:0] gcnt);
+    reg [7:0] c;
+    reg [3:0] s;
+    reg [7:0] g;
+    wire gclk = clk & en;
+    always @(posedge clk) begin
+        c <= c + 8'd1;
+        s <= {s[2:0], d[0]};
+    end
+    always @(posedge gclk) g <= g + {4'd0, d};
+    assign cnt = c;
+    assign sr = s;
+    assign gcnt = g;
+endmodule
+"""
+
+
+def gated_closed_form(stim, seed: int, sid: int, cycles: int) -> list:
+    """Per-cycle output bytes (cnt, sr, gcnt) of ``gated`` for stimulus
+    ``sid``; ``stim(seed, s, c, k)`` is the stimulus generator."""
+    c = s = g = 0
+    out = []
+    for cy in range(cycles):
+        en = stim(seed, sid, cy, 0) & 1
+        d = stim(seed, sid, cy, 1 << 3) & 15
+        c = (c + 1) & 255
+        s = ((s << 1) | (d & 1)) & 15
+        if en:
+            g = (g + d) & 255
+        out.append(bytes([c, s, g]))
+    return out

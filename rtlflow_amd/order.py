@@ -280,9 +280,11 @@ def build_schedule(design: Design, keep: Optional[Set[str]] = None) -> Schedule:
         else:
             raise CompileError(f"{v.name} is driven by both clocked and combinational logic")
     for c in clocks:
-        if c.kind != "input":
-            raise CompileError(f"clock {c.name} must be a primary input (derived clocks "
-                               "are not supported)")
+        # a derived (gated) clock must be combinational logic of inputs and
+        # state; one that a clocked block drives (a divider) is not modelled
+        if c.kind != "input" and sch.roles.get(c) != "comb":
+            raise CompileError(f"clock {c.name} must be a primary input or combinational "
+                               "logic (clocks driven by clocked logic are not supported)")
 
     # ---- dead-logic elimination (V3Dead / V3Life role): keep what reaches an output
     observed = set(design.outputs)
@@ -413,14 +415,37 @@ def build_schedule(design: Design, keep: Optional[Set[str]] = None) -> Schedule:
         cacc = sch.access[c.id]
         for v in cacc.dep_reads | cacc.need_old:
             comb_readers.setdefault(v, []).append(c)
+    # A derived clock (combinational logic of inputs, such as a gated clock)
+    # is computed before the clocked blocks it triggers, so their edge test
+    # sees its new value; its logic may not read clocked state (a clock that
+    # state gates could glitch within the edge's time step).
+    derived = {c for c in design.clocks if c.kind != "input"}
+    cone: Set[int] = set()
+    work = [w.id for c in derived for w in comb_writers.get(c, [])]
+    while work:
+        i = work.pop()
+        if i in cone:
+            continue
+        cone.add(i)
+        for v in sch.access[i].dep_reads | sch.access[i].need_old:
+            if sch.roles.get(v) == "state":
+                raise CompileError(f"derived clock logic reads clocked state {v.name} "
+                                   "(gated by a register; not supported)")
+            work += [w.id for w in comb_writers.get(v, [])]
+    if any(len(comp) > 1 and cone & set(comp) for comp in comb_sccs):
+        raise CompileError("derived clock logic in a combinational cycle")
     for s in seq:
         sacc = sch.access[s.id]
+        eclk = {clk for _, clk in s.edges if clk in derived}
         for v in sacc.writes:
             for c in comb_readers.get(v, ()):
                 add_edge(proc2vx[s.id], proc2vx[c.id])
-        for v in sacc.dep_reads | sacc.need_old:
+        for v in (sacc.dep_reads | sacc.need_old) - eclk:
             for w in comb_writers.get(v, []):
                 add_edge(proc2vx[s.id], proc2vx[w.id])
+        for v in eclk:
+            for w in comb_writers.get(v, []):
+                add_edge(proc2vx[w.id], proc2vx[s.id])
 
     sch.vertices = vertices
     sch.change_vars = sorted(change, key=lambda v: v.id)

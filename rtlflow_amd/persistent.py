@@ -106,7 +106,7 @@ def emit_persistent(comp) -> str:
     gen.emit("#pragma unroll 1")
     gen.emit("for (uint32_t ph = 0u; ph < 2u; ++ph) {")
     gen.ind += 1
-    for clk in d.clocks:
+    for clk in comp.in_clocks:
         gen.emit(gen.set(clk, "ph"))
         gen.dirty.add(clk)
     _eval_loop_body(comp, gen)

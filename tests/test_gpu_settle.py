@@ -82,3 +82,34 @@ def test_unsettled_loop_reports(gpu, mode):
         m.set_port("clk", np.zeros(64, np.uint8))
         with pytest.raises(RuntimeError, match="converge"):
             m.eval()
+
+
+def _gated_want(seed, n, cycles, budgets=None):
+    from rtlflow_amd.designs import settle
+    out = np.zeros(n, np.uint64)
+    for s in range(n):
+        h = 0xcbf29ce484222325
+        cyc = cycles if budgets is None else int(budgets[s])
+        for b in settle.gated_closed_form(oracle.stim, seed, s, cyc):
+            h = oracle.fnv(b, h)
+        out[s] = h
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [MODE_GRAPH, MODE_FUSED])
+def test_gated_clock(gpu, mode):
+    """A gated clock (clk & en): the counter it clocks advances exactly in
+    the cycles whose enable is set, in the graph and fused forms and the
+    batched queue."""
+    cd = registry.get("gated")
+    seed, n, cycles = 0x6A7E, 100, 40
+    with cd.model(n) as m:
+        m.begin(seed)
+        m.run_cycles(cycles, mode)
+        got = m.hashes()
+    assert (got == _gated_want(seed, n, cycles)).all()
+    budgets = np.array([40, 0, 5, 17, 33] * 6, np.uint32)
+    with cd.model(8) as m:
+        got = m.submit_queue(seed, budgets)
+    assert (got == _gated_want(seed, len(budgets), 0, budgets)).all()

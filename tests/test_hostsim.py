@@ -145,3 +145,29 @@ def test_settle_loop_host_build():
     assert got == [_circ_closed_form(seed, 1, b, stim0=i)[0] for i, b in enumerate(budgets)]
     with pytest.raises(RuntimeError):
         HostDesign(Compiled(elaborate(settle.OSC), Options(threads=2))).protocol(2, seed, 3, mode=1)
+
+
+def test_gated_clock_host_build():
+    """A derived (gated) clock, clk & en: its logic runs before the block it
+    clocks, whose edge test sees the new value; closed form, interpreter and
+    every schedule form agree. A clock gated by a register is refused."""
+    import ir_interp
+    from rtlflow_amd.designs import settle
+    from rtlflow_amd.order import CompileError
+    seed, n, cycles = 0x6A7E, 5, 30
+    want = []
+    for s in range(n):
+        h = 0xcbf29ce484222325
+        for b in settle.gated_closed_form(oracle.stim, seed, s, cycles):
+            h = oracle.fnv(b, h)
+        want.append(h)
+    assert ir_interp.protocol(elaborate(settle.GATED), seed, n, cycles) == want
+    hd = HostDesign(Compiled(elaborate(settle.GATED), Options(threads=2)))
+    for mode in (0, 1, 2):
+        assert list(hd.protocol(n, seed, cycles, mode=mode, order_seed=5)) == want
+    with pytest.raises(CompileError, match="clocked state"):
+        Compiled(elaborate("""module g2(input clk, input en, output [3:0] o);
+            reg e; reg [3:0] q; wire gc = clk & e;
+            always @(posedge clk) e <= en;
+            always @(posedge gc) q <= q + 4'd1;
+            assign o = q; endmodule"""), Options(threads=1))
