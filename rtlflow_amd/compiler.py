@@ -14,7 +14,7 @@ from typing import Callable, Dict, List, Optional, Set
 
 from .emit_hip import ARR, CTY, Gen, cty, edge_cond, is_wide, lit
 from .instrcount import process_cost
-from .ir import CLASS_BYTES, Design, Ref, Var, words_of
+from .ir import CLASS_BYTES, Assign, Design, Ref, Var, words_of
 from .layout import Layout, class_of
 from .order import CompileError, Schedule, build_schedule, split_seq_processes
 from .partition import MTask, partition, partition_stats
@@ -328,6 +328,26 @@ def _vertex_vars(comp: Compiled, vids: List[int]) -> Set[Var]:
     return {v for v in out if not v.is_mem}
 
 
+def _nb_temps_needed(sch, procs) -> Set[Var]:
+    """Non-blocking targets of a clocked group that need a temporary: those
+    read (for their pre-edge value) after the group writes them. Processes
+    run in list order; a process that is one assignment reads its right-hand
+    side before it writes, any other process may read its own targets
+    afterwards."""
+    first: Dict[Var, int] = {}
+    for k, p in enumerate(procs):
+        for v in sch.access[p.id].nb_writes:
+            first.setdefault(v, k)
+    need: Set[Var] = set()
+    for k, p in enumerate(procs):
+        acc = sch.access[p.id]
+        single = len(p.stmts) == 1 and isinstance(p.stmts[0], Assign)
+        for v in acc.dep_reads | acc.need_old:
+            if v in first and (first[v] < k or (first[v] == k and not single)):
+                need.add(v)
+    return need
+
+
 def _emit_vertices(comp: Compiled, gen: Gen, vids: List[int], store_in_guard: bool):
     sch = comp.sch
     for vid in vids:
@@ -343,11 +363,7 @@ def _emit_vertices(comp: Compiled, gen: Gen, vids: List[int], store_in_guard: bo
             gen.ind += 1
             saved = set(gen.avail)
             saved_dirty = set(gen.dirty)
-            # NBA targets read anywhere in this clocked group need a temporary
-            group_reads = set()
-            for p in vx.procs:
-                group_reads |= sch.access[p.id].dep_reads
-            gen.nb_needed = group_reads
+            gen.nb_needed = _nb_temps_needed(sch, vx.procs)
             gen.seq_group(vx.procs, [sch.access[p.id] for p in vx.procs], f"{vx.procs[0].id}")
             gen.nb_needed = None
             if store_in_guard:
