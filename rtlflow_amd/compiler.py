@@ -783,6 +783,81 @@ def emit_init(comp: Compiled) -> str:
     return "\n".join(lines)
 
 
+def clk0_cone(comp: Compiled) -> Optional[List[int]]:
+    """The vertices a cycle's clk = 0 eval must run once the design has
+    settled: the combinational logic that reads the inputs (new this cycle)
+    or the clock, directly or through other such logic. Nothing clocked
+    fires on the falling clock and the state has not changed since the last
+    settle, so every other value still holds. None when that does not hold
+    (negedge blocks, derived clocks, combinational cycles)."""
+    sch = comp.sch
+    d = comp.design
+    if sch.change_vars or any(c.kind != "input" for c in d.clocks):
+        return None
+    for vx in sch.vertices:
+        if vx.kind == "seq" and any(e != "posedge" for p in vx.procs for e, _ in p.edges):
+            return None
+    # logic placed before a writer of what it reads (the combinational
+    # inputs of clocked blocks that also read their state) last ran on the
+    # old value
+    pos = {vid: k for k, vid in enumerate(sch.topo)}
+    last_write: Dict[Var, int] = {}
+    for vid in sch.topo:
+        for v in sch.vertices[vid].writes:
+            last_write[v] = pos[vid]
+    changed: Set[Var] = set(d.inputs)
+    out = []
+    for vid in sch.topo:
+        vx = sch.vertices[vid]
+        if vx.kind != "comb":
+            continue
+        reads = vx.reads | vx.need_old
+        if reads & changed or any(last_write.get(v, -1) > pos[vid] for v in reads):
+            out.append(vid)
+            changed |= vx.writes
+    return out
+
+
+def clk0_carried(comp: Compiled, cone: Optional[List[int]]) -> Set[Var]:
+    """Values the clk = 0 eval reads without computing them (written by
+    logic outside ``cone``): they must keep their value from the previous
+    eval, so they cannot share a register slot with shorter-lived values."""
+    if cone is None:
+        return set()
+    sch = comp.sch
+    inside = set(cone)
+    read: Set[Var] = set()
+    for vid in cone:
+        read |= sch.vertices[vid].reads | sch.vertices[vid].need_old
+    out: Set[Var] = set()
+    for vx in sch.vertices:
+        if vx.id not in inside:
+            out |= vx.writes & read
+    return out
+
+
+def _eval_loop_body_split(comp: Compiled, gen: Gen, first: str, cone: Optional[List[int]]):
+    """The two evals of a cycle (``ph`` = the clock): the clk = 0 eval runs
+    only ``cone`` (clk0_cone()) except when ``first`` (the first cycle a
+    kernel runs for a lane, whose combinational values it has not computed
+    yet)."""
+    if cone is None:
+        _eval_loop_body(comp, gen)
+        return
+    gen.emit(f"if (ph == 0u && !({first})) {{")
+    gen.ind += 1
+    _emit_vertices(comp, gen, cone, store_in_guard=False)
+    for clk, last in comp.clklast.items():
+        gen.emit(gen.set(last, gen.get(clk)))
+        gen.dirty.add(last)
+    gen.ind -= 1
+    gen.emit("} else {")
+    gen.ind += 1
+    _eval_loop_body(comp, gen)
+    gen.ind -= 1
+    gen.emit("}")
+
+
 def _eval_loop_body(comp: Compiled, gen: Gen):
     """One eval() with the settle loop on device (src/V3EmitC.cpp:5426-5496)."""
     all_vids = list(comp.sch.topo)
@@ -983,7 +1058,8 @@ def emit_fused(comp: Compiled) -> str:
     gen.lds = lds_memories(comp)
     gen.top_lines = []
     gen.local = _fused_locals(comp)
-    plan_packing(comp, gen, comp.stored)
+    cone = clk0_cone(comp)
+    plan_packing(comp, gen, comp.stored | clk0_carried(comp, cone))
     lds_state = place_lds_state(comp, gen)
     # registers that carry values across evals are loaded once
     for v in sorted(gen.local, key=lambda x: x.id):
@@ -1004,7 +1080,7 @@ def emit_fused(comp: Compiled) -> str:
     for clk in comp.in_clocks:
         gen.emit(gen.set(clk, "ph"))
         gen.dirty.add(clk)
-    _eval_loop_body(comp, gen)
+    _eval_loop_body_split(comp, gen, "n == 0u", cone)
     gen.ind -= 1
     gen.emit("}")
     _hash_code(comp, gen, "h")

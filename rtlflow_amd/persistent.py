@@ -15,7 +15,7 @@ rule, which bounds the tail by the longest single stimulus).
 """
 from __future__ import annotations
 
-from .compiler import (_eval_loop_body, _fused_locals, _hash_code, _initial_body, _lds_copy_lines,
+from .compiler import (_eval_loop_body_split, _fused_locals, clk0_carried, clk0_cone, _hash_code, _initial_body, _lds_copy_lines,
                        _lds_decl_lines, _prologue, _stim_code, atomic, kernel_header, lds_memories,
                        place_lds_state, plan_packing)
 from .emit_hip import Gen, is_wide
@@ -67,7 +67,8 @@ def emit_persistent(comp) -> str:
     gen.lds = lds_memories(comp)
     gen.top_lines = []
     gen.local = _fused_locals(comp)
-    plan_packing(comp, gen, comp.stored)
+    cone = clk0_cone(comp)
+    plan_packing(comp, gen, comp.stored | clk0_carried(comp, cone))
     lds_state = place_lds_state(comp, gen)
     for v in sorted(gen.local, key=lambda x: x.id):
         gen.declare(v)
@@ -109,7 +110,7 @@ def emit_persistent(comp) -> str:
     for clk in comp.in_clocks:
         gen.emit(gen.set(clk, "ph"))
         gen.dirty.add(clk)
-    _eval_loop_body(comp, gen)
+    _eval_loop_body_split(comp, gen, "cyc == 0u", cone)
     gen.ind -= 1
     gen.emit("}")
     _hash_code(comp, gen, "h")
