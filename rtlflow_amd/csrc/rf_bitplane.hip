@@ -68,6 +68,40 @@ __device__ __forceinline__ uint64_t bp_gather(const uint64_t* L, const uint32_t*
 // a slot's LDS byte offset (slot * 8), with the inversion flags in bit 0 of
 // the x and y fields: x = dst8 | s8 << 16; y = (xs8 | ix) | (ys8 | iy) << 16.
 // The byte form needs no shift per operand address.
+// The gate on decoded byte-form fields (LDS byte offsets; ix, iy 0 or ~0).
+__device__ __forceinline__ void bp_gate_bytes(uint32_t dst, uint32_t s, uint32_t xs, uint32_t ys, uint32_t ix,
+                                              uint32_t iy) {
+    const uint64_t S = *reinterpret_cast<const lds_u64*>((size_t)s);
+    const uint64_t X = *reinterpret_cast<const lds_u64*>((size_t)xs);
+    const uint64_t Y = *reinterpret_cast<const lds_u64*>((size_t)ys);
+    const uint32_t xl = (uint32_t)X ^ ix, xh = (uint32_t)(X >> 32) ^ ix;
+    const uint32_t yl = (uint32_t)Y ^ iy, yh = (uint32_t)(Y >> 32) ^ iy;
+    const uint32_t sl = (uint32_t)S, sh = (uint32_t)(S >> 32);
+    *reinterpret_cast<lds_u64*>((size_t)dst) =
+        ((uint64_t)((sh & xh) | (~sh & yh)) << 32) | (uint64_t)((sl & xl) | (~sl & yl));
+}
+
+// A register-resident byte-form record, decoded by an opaque instruction
+// sequence: the compiler can neither hoist the six decoded fields out of
+// the cycle loop (six registers per record instead of two) nor needs a copy
+// of the record to keep it intact. The empty record (x = 0xffffffff, no
+// valid destination offset 0xffff) is tested on the decoded field for the
+// same reason: a test of the record itself would be hoisted as a lane mask
+// per record.
+__device__ __forceinline__ void bp_gate_bytes_live(uint2 w) {
+    uint32_t dst, s, xs, ys, ix, iy;
+    asm volatile(
+        "v_and_b32 %0, 0xffff, %6\n\t"
+        "v_lshrrev_b32 %1, 16, %6\n\t"
+        "v_and_b32 %2, 0xfff8, %7\n\t"
+        "v_and_b32_sdwa %3, %7, %8 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:WORD_1 src1_sel:DWORD\n\t"
+        "v_bfe_i32 %4, %7, 0, 1\n\t"
+        "v_bfe_i32 %5, %7, 16, 1"
+        : "=&v"(dst), "=&v"(s), "=&v"(xs), "=&v"(ys), "=&v"(ix), "=&v"(iy)
+        : "v"(w.x), "v"(w.y), "s"(0xfff8u));
+    if (dst != 0xffffu) bp_gate_bytes(dst, s, xs, ys, ix, iy);
+}
+
 template <bool BA>
 __device__ __forceinline__ void bp_gate(uint64_t* L, uint2 w) {
     if constexpr (BA) {
@@ -81,14 +115,7 @@ __device__ __forceinline__ void bp_gate(uint64_t* L, uint2 w) {
         const uint32_t ys = (w.y >> 16) & 0xfff8u;
         const uint32_t ix = (uint32_t)((int32_t)(w.y << 31) >> 31);   // 0 or ~0
         const uint32_t iy = (uint32_t)((int32_t)(w.y << 15) >> 31);
-        const uint64_t S = *reinterpret_cast<const lds_u64*>((size_t)s);
-        const uint64_t X = *reinterpret_cast<const lds_u64*>((size_t)xs);
-        const uint64_t Y = *reinterpret_cast<const lds_u64*>((size_t)ys);
-        const uint32_t xl = (uint32_t)X ^ ix, xh = (uint32_t)(X >> 32) ^ ix;
-        const uint32_t yl = (uint32_t)Y ^ iy, yh = (uint32_t)(Y >> 32) ^ iy;
-        const uint32_t sl = (uint32_t)S, sh = (uint32_t)(S >> 32);
-        *reinterpret_cast<lds_u64*>((size_t)dst) =
-            ((uint64_t)((sh & xh) | (~sh & yh)) << 32) | (uint64_t)((sl & xl) | (~sl & yl));
+        bp_gate_bytes(dst, s, xs, ys, ix, iy);
         return;
     }
     const uint32_t dst = w.x & 0xffffu;
@@ -277,18 +304,26 @@ __device__ __forceinline__ void bp_commit(uint64_t* L, const uint32_t* c, uint32
 // Rounds [i0, i1) of the register-resident gate records, a barrier where
 // the round table says a level ends.
 template <int MAXR, bool BA>
-__device__ __forceinline__ void bp_rounds(uint64_t* L, const uint2 (&d)[MAXR], const rf_bp_params& P,
-                                          int i0, int i1) {
+__device__ __forceinline__ void bp_rounds(uint64_t* L, const uint2 (&d)[MAXR], const rf_bp_params& P) {
+    // the round count and barrier mask pass an opaque step each cycle, so
+    // the per-round tests stay scalar compares here instead of precomputed
+    // lane masks (which outgrow the scalar registers and spill)
+    uint32_t nr = P.nr;
+    uint64_t bar = P.bar;
+    __asm__ volatile("" : "+s"(nr), "+s"(bar));
 #pragma unroll
     for (int i = 0; i < MAXR; ++i) {
-        if (i >= i0 && i < i1) {
-            // an opaque copy: keeps the compiler from hoisting the decoded
-            // fields of every record out of the cycle loop (six registers
-            // per record instead of two)
-            uint2 w = d[i];
-            __asm__ volatile("" : "+v"(w.x), "+v"(w.y));
-            if (w.x != 0xffffffffu) bp_gate<BA>(L, w);
-            if ((P.bar >> i) & 1ull) __syncthreads();
+        if (i < (int)nr) {
+            if constexpr (BA) {
+                bp_gate_bytes_live(d[i]);
+            } else {
+                // an opaque copy: keeps the compiler from hoisting the
+                // decoded fields of every record out of the cycle loop
+                uint2 w = d[i];
+                __asm__ volatile("" : "+v"(w.x), "+v"(w.y));
+                if (w.x != 0xffffffffu) bp_gate<BA>(L, w);
+            }
+            if ((bar >> i) & 1ull) __syncthreads();
         }
     }
 }
@@ -424,7 +459,7 @@ rf_bp_kernel(rf_bp_params P) {
         // cycle n's outputs and cycle n + 1's next state in one pass
         for (uint32_t n = 0; n + 1 < N; ++n) {
             if constexpr (MAXR > 0)
-                bp_rounds<MR, BA>(L, d, P, 0, (int)P.nr);
+                bp_rounds<MR, BA>(L, d, P);
             else
                 bp_phase<NT, BA>(L, lev[RF_BP_M], gates[RF_BP_M], P.nlev[RF_BP_M], tid);
             bp_commit<NT>(L, prog + P.off_commit[1], P.n_commit[1], tid);
