@@ -41,6 +41,12 @@ struct rf_bp_params {
     uint32_t off_in, off_state, off_commit[2], off_hash[2], off_f;
     uint32_t off_mem, off_ports[2];   // memory table; port lists after A0 and M
     uint32_t nlev[3], off_lev[3], off_gates[3];
+    uint32_t hash_words;        // words of program M's hashed-output table
+    // LDS byte offset of the per-cycle tables (program M's commit pairs
+    // packed as state | next << 16, its hashed-output table, the input
+    // chunks) when the launch holds them after the planes; 0: read from
+    // ``prog``. Set when the extra LDS leaves the workgroups per CU unchanged.
+    uint32_t tab_off;
 };
 
 // Parse and check a blob (every slot index, level table and section size);

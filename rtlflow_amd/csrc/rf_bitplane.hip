@@ -84,8 +84,8 @@ __device__ __forceinline__ void bp_to_planes(uint64_t* L, uint32_t s0, uint64_t 
 }
 
 // This lane's bits of planes refs[0 .. nbits) (nbits <= 64).
-__device__ __forceinline__ uint64_t bp_gather(const uint64_t* L, const uint32_t* refs, uint32_t nbits,
-                                              uint32_t t) {
+template <typename TP>
+__device__ __forceinline__ uint64_t bp_gather(const uint64_t* L, TP refs, uint32_t nbits, uint32_t t) {
     const uint64_t row = t < nbits ? bp_ref(L, refs[t]) : 0ull;
     return bp_transpose64(row, t);
 }
@@ -208,33 +208,38 @@ __device__ __forceinline__ void bp_store(const rf_args& a, uint32_t cls, uint64_
     }
 }
 
+// The per-cycle tables (input chunks, hashed outputs of program M, commit
+// pairs of program M) are read from global memory, or from the copy the
+// kernel makes in LDS after the planes (lds_u32 pointers) when the launch
+// reserves room for it.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
 // Inputs of one cycle into their planes (wavefront 0): set A, read by the
 // clk = 0 settle.
-__device__ __forceinline__ void bp_inputs(uint64_t* L, const rf_bp_params& P, uint64_t sid, uint32_t cyc,
-                                          uint32_t t) {
-    const uint32_t* ic = P.prog + P.off_in;
-    for (uint32_t k = 0; k < P.n_in; ++k, ic += 4) {
-        const uint64_t r = rf_stim(P.a.seed, sid, (uint64_t)cyc, ic[0]);
+template <typename TP>
+__device__ __forceinline__ void bp_inputs(uint64_t* L, TP ic, uint32_t n_in, uint64_t seed, uint64_t sid,
+                                          uint32_t cyc, uint32_t t) {
+    for (uint32_t k = 0; k < n_in; ++k, ic += 4) {
+        const uint64_t r = rf_stim(seed, sid, (uint64_t)cyc, ic[0]);
         bp_to_planes(L, ic[2], r, ic[1], t);
     }
 }
 
 // Input set A -> set B (the clk = 1 settle of the same cycle reads set B
 // while set A already holds the next cycle's inputs); wavefront 0.
-__device__ __forceinline__ void bp_copy_inputs(uint64_t* L, const rf_bp_params& P, uint32_t t) {
-    const uint32_t* ic = P.prog + P.off_in;
-    for (uint32_t k = 0; k < P.n_in; ++k, ic += 4)
+template <typename TP>
+__device__ __forceinline__ void bp_copy_inputs(uint64_t* L, TP ic, uint32_t n_in, uint32_t t) {
+    for (uint32_t k = 0; k < n_in; ++k, ic += 4)
         if (t < ic[1]) L[ic[3] + t] = L[ic[2] + t];
 }
 
-// Fold this cycle's outputs into the lane's hash (wavefront 1); ``which``
-// selects the output-slot table of program M or B.
-__device__ __forceinline__ uint64_t bp_hash(const uint64_t* L, const rf_bp_params& P, uint32_t which,
-                                            uint64_t h, uint32_t t) {
-    const uint32_t* hp = P.prog + P.off_hash[which];
-    for (uint32_t k = 0; k < P.n_hash; ++k) {
+// Fold this cycle's outputs into the lane's hash (wavefront 1); ``hp``: the
+// output-slot table of program M or B.
+template <typename TP>
+__device__ __forceinline__ uint64_t bp_hash(const uint64_t* L, TP hp, uint32_t n_hash, uint64_t h, uint32_t t) {
+    for (uint32_t k = 0; k < n_hash; ++k) {
         const uint32_t nbytes = hp[0], width = hp[1];
-        const uint32_t* refs = hp + 2;
+        const TP refs = hp + 2;
         if (width <= 64) {
             h = rf_fnv_bytes(h, bp_gather(L, refs, width, t), (int)nbytes);
         } else {
@@ -326,6 +331,15 @@ __device__ __forceinline__ void bp_mem_ports(uint64_t* L, const rf_bp_params& P,
 template <int NT>
 __device__ __forceinline__ void bp_commit(uint64_t* L, const uint32_t* c, uint32_t n, uint32_t tid) {
     for (uint32_t i = tid; i < n; i += NT) L[c[2 * i]] = L[c[2 * i + 1]];
+}
+
+// Commit pairs packed as state slot | next-state slot << 16, in LDS.
+template <int NT>
+__device__ __forceinline__ void bp_commit_lds(uint64_t* L, const lds_u32* c, uint32_t n, uint32_t tid) {
+    for (uint32_t i = tid; i < n; i += NT) {
+        const uint32_t e = c[i];
+        L[e & 0xffffu] = L[e >> 16];
+    }
 }
 
 // Rounds [i0, i1) of the register-resident gate records, a barrier where
@@ -449,7 +463,22 @@ rf_bp_kernel(rf_bp_params P) {
             }
         }
     }
-    if (w0 && a.ncycles) bp_inputs(L, P, sid, cyc, t);
+    const uint32_t* in_tab = prog + P.off_in;
+    if (w0 && a.ncycles) bp_inputs(L, in_tab, P.n_in, a.seed, sid, cyc, t);
+    // per-cycle tables into LDS after the planes
+    const lds_u32* tc = reinterpret_cast<const lds_u32*>((size_t)P.tab_off);
+    const lds_u32* th = tc + P.n_commit[1];
+    const lds_u32* ti = th + P.hash_words;
+    if (P.tab_off) {
+        lds_u32* w = reinterpret_cast<lds_u32*>((size_t)P.tab_off);
+        const uint32_t* cm = prog + P.off_commit[1];
+        for (uint32_t i = tid; i < P.n_commit[1]; i += NT) w[i] = cm[2 * i] | (cm[2 * i + 1] << 16);
+        w += P.n_commit[1];
+        const uint32_t* hp = prog + P.off_hash[0];
+        for (uint32_t i = tid; i < P.hash_words; i += NT) w[i] = hp[i];
+        w += P.hash_words;
+        for (uint32_t i = tid; i < 4u * P.n_in; i += NT) w[i] = in_tab[i];
+    }
     __syncthreads();
     const uint32_t* lev[3];
     const uint2* gates[3];
@@ -479,8 +508,8 @@ rf_bp_kernel(rf_bp_params P) {
         bp_commit<NT>(L, prog + P.off_commit[0], P.n_commit[0], tid);
         if (P.n_mem) bp_mem_ports(L, P, 0, wave, NW, t, lane, inb, act);
         if (w0) {
-            bp_copy_inputs(L, P, t);
-            if (N > 1) bp_inputs(L, P, sid, cyc + 1, t);
+            bp_copy_inputs(L, in_tab, P.n_in, t);
+            if (N > 1) bp_inputs(L, in_tab, P.n_in, a.seed, sid, cyc + 1, t);
         }
         __syncthreads();
         // cycle n's outputs and cycle n + 1's next state in one pass
@@ -489,21 +518,30 @@ rf_bp_kernel(rf_bp_params P) {
                 bp_rounds<MR, BA>(L, d, P);
             else
                 bp_phase<NT, BA>(L, lev[RF_BP_M], gates[RF_BP_M], P.nlev[RF_BP_M], tid);
-            bp_commit<NT>(L, prog + P.off_commit[1], P.n_commit[1], tid);
+            if (P.tab_off)
+                bp_commit_lds<NT>(L, tc, P.n_commit[1], tid);
+            else
+                bp_commit<NT>(L, prog + P.off_commit[1], P.n_commit[1], tid);
             if (P.n_mem) bp_mem_ports(L, P, 1, wave, NW, t, lane, inb, act);
             if (w1) {
-                const uint64_t hn = bp_hash(L, P, 0, h, t);
+                const uint64_t hn = P.tab_off ? bp_hash(L, th, P.n_hash, h, t)
+                                              : bp_hash(L, prog + P.off_hash[0], P.n_hash, h, t);
                 if (act && n < lim) h = hn;
             } else if (w0) {
-                bp_copy_inputs(L, P, t);
-                if (n + 2 < N) bp_inputs(L, P, sid, cyc + n + 2, t);
+                if (P.tab_off) {
+                    bp_copy_inputs(L, ti, P.n_in, t);
+                    if (n + 2 < N) bp_inputs(L, ti, P.n_in, a.seed, sid, cyc + n + 2, t);
+                } else {
+                    bp_copy_inputs(L, in_tab, P.n_in, t);
+                    if (n + 2 < N) bp_inputs(L, in_tab, P.n_in, a.seed, sid, cyc + n + 2, t);
+                }
             }
             __syncthreads();
         }
         // the last cycle's outputs
         bp_phase<NT, BA>(L, lev[RF_BP_B], gates[RF_BP_B], P.nlev[RF_BP_B], tid);
         if (w1) {
-            const uint64_t hn = bp_hash(L, P, 1, h, t);
+            const uint64_t hn = bp_hash(L, prog + P.off_hash[1], P.n_hash, h, t);
             if (act && N - 1 < lim) h = hn;
         }
         __syncthreads();
@@ -662,6 +700,7 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
             if (!bp_check_refs(b + o + 2, width, p->nslots)) BP_FAIL("bit-plane hash slot out of range");
             o += 2ull + width;
         }
+        if (which == 0) p->hash_words = (uint32_t)(o - p->off_hash[0]);
     }
     o = ev(o);
     p->n_mem = b[17];
@@ -746,6 +785,12 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
             const uint32_t* c = b + p->off_commit[which] + 2 * k;
             if (c[0] >= p->nslots || c[1] >= p->nslots) BP_FAIL("bit-plane commit slot out of range");
         }
+    // the per-cycle tables go to LDS after the planes when that costs no
+    // workgroup per CU (160 KB of LDS per CU)
+    const uint64_t planes = (uint64_t)p->nslots * 8u;
+    const uint64_t tab = 4ull * (p->n_commit[1] + (uint64_t)p->hash_words + 4ull * p->n_in);
+    const uint64_t cu = 160u * 1024u;
+    p->tab_off = (planes + tab <= cu - 64u && cu / planes == cu / (planes + tab)) ? (uint32_t)planes : 0u;
     return 0;
 #undef BP_FAIL
 }
@@ -772,9 +817,13 @@ void rf_bp_rounds(const uint32_t* b, uint32_t threads, rf_bp_params* p) {
     p->nr = r;
 }
 
+static uint32_t bp_tab_words(const rf_bp_params& p) {
+    return p.n_commit[1] + p.hash_words + 4u * p.n_in;
+}
+
 template <int NT, int MAXR, bool BA>
 static hipError_t bp_launch_ba(const rf_bp_params& p, hipStream_t stream) {
-    const size_t shmem = (size_t)p.nslots * 8u;
+    const size_t shmem = (size_t)p.nslots * 8u + (p.tab_off ? 4u * (size_t)bp_tab_words(p) : 0u);
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rf_bp_kernel<NT, MAXR, BA>),
