@@ -333,10 +333,14 @@ __device__ __forceinline__ void bp_commit(uint64_t* L, const uint32_t* c, uint32
     for (uint32_t i = tid; i < n; i += NT) L[c[2 * i]] = L[c[2 * i + 1]];
 }
 
-// Commit pairs packed as state slot | next-state slot << 16, in LDS.
+// Commit pairs packed as state slot | next-state slot << 16, in LDS. With
+// more than two wavefronts, wavefronts 0 and 1 take no pairs: they start
+// their lane work (inputs, hash) at once, which runs beside the commit.
 template <int NT>
 __device__ __forceinline__ void bp_commit_lds(uint64_t* L, const lds_u32* c, uint32_t n, uint32_t tid) {
-    for (uint32_t i = tid; i < n; i += NT) {
+    constexpr uint32_t SKIP = NT > 128 ? 128u : 0u;
+    if (tid < SKIP) return;
+    for (uint32_t i = tid - SKIP; i < n; i += NT - SKIP) {
         const uint32_t e = c[i];
         L[e & 0xffffu] = L[e >> 16];
     }
