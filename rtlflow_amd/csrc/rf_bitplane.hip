@@ -129,6 +129,27 @@ __device__ __forceinline__ void bp_gate_bytes_live(uint2 w) {
     if (dst != 0xffffu) bp_gate_bytes(dst, s, xs, ys, ix, iy);
 }
 
+// The slot form's register-resident record, decoded the same way into LDS
+// byte offsets (the planes start at LDS address 0): dst and s are the two
+// halves of x, shifted; xs and ys drop the inversion flags (bits 15 and 31
+// of y). The empty record's destination half 0xffff becomes 0x7fff8, which
+// no slot has (at most 0x7fff slots).
+__device__ __forceinline__ void bp_gate_slots_live(uint2 w) {
+    uint32_t dst, s, xs, ys, ix, iy;
+    asm volatile(
+        "v_lshlrev_b32_sdwa %0, %8, %6 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_0\n\t"
+        "v_lshlrev_b32_sdwa %1, %8, %6 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_1\n\t"
+        "v_lshlrev_b32 %2, 3, %7\n\t"
+        "v_and_b32 %2, %9, %2\n\t"
+        "v_lshrrev_b32 %3, 13, %7\n\t"
+        "v_and_b32 %3, %9, %3\n\t"
+        "v_bfe_i32 %4, %7, 15, 1\n\t"
+        "v_ashrrev_i32 %5, 31, %7"
+        : "=&v"(dst), "=&v"(s), "=&v"(xs), "=&v"(ys), "=&v"(ix), "=&v"(iy)
+        : "v"(w.x), "v"(w.y), "s"(3u), "s"(0x3fff8u));
+    if (dst != 0x7fff8u) bp_gate_bytes(dst, s, xs, ys, ix, iy);
+}
+
 template <bool BA>
 __device__ __forceinline__ void bp_gate(uint64_t* L, uint2 w) {
     if constexpr (BA) {
@@ -362,11 +383,7 @@ __device__ __forceinline__ void bp_rounds(uint64_t* L, const uint2 (&d)[MAXR], c
             if constexpr (BA) {
                 bp_gate_bytes_live(d[i]);
             } else {
-                // an opaque copy: keeps the compiler from hoisting the
-                // decoded fields of every record out of the cycle loop
-                uint2 w = d[i];
-                __asm__ volatile("" : "+v"(w.x), "+v"(w.y));
-                if (w.x != 0xffffffffu) bp_gate<BA>(L, w);
+                bp_gate_slots_live(d[i]);
             }
             if ((bar >> i) & 1ull) __syncthreads();
         }
