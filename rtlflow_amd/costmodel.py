@@ -17,8 +17,8 @@ of a vertex or an mtask to its measured kernel time:
 Features enter as log1p, the target as log(microseconds), so the network
 models multiplicative effects. Training data come from the simulator itself:
 every mtask kernel of a set of synthetic designs is timed in isolation on the
-GPU (rf_profile_mtasks). The heuristic remains the fallback and the judge: a
-learned schedule is kept only if its measured throughput is higher and its
+GPU (rf_profile_mtasks). The heuristic remains the fallback and the reference
+point: a learned schedule is kept only if its measured throughput is higher and its
 hashes are identical.
 """
 from __future__ import annotations

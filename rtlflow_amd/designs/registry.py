@@ -2,7 +2,7 @@
 
 ``build_all()`` (called by __graft_entry__.build on the CPU) generates,
 compiles and links every entry and records it in ``_build/designs/index.json``;
-``get(name)`` on the GPU box only reads that index and loads the prebuilt
+``get(name)`` on a GPU machine only reads that index and loads the prebuilt
 library -- it never compiles. Each entry also keeps its generator inputs so
 the CPU oracle can be driven with exactly the same design.
 """
