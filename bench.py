@@ -68,13 +68,16 @@ def run_form(cd, lanes, mode, steps, cps, warmup, seed, chunk, bp_threads=0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cycles-per-step", type=int, default=500)
+    ap.add_argument("--cycles-per-step", type=int, default=2000,
+                    help="cycles per step (default: one step is the whole 2,000-cycle workload)")
     ap.add_argument("--lanes", type=int, default=65536)
     ap.add_argument("--design", default="net2")
     ap.add_argument("--forms", default="bitplane,fused,graph")
-    ap.add_argument("--chunk", type=int, default=512, help="cycles per fused / bit-plane launch")
+    ap.add_argument("--chunk", type=int, default=512, help="cycles per fused launch")
+    ap.add_argument("--bp-chunk", type=int, default=2048,
+                    help="cycles per bit-plane launch (the state stays in LDS across them)")
     ap.add_argument("--learned-cycles", type=int, default=400,
                     help="cycles of the learned-vs-heuristic partition comparison (0: skip)")
     ap.add_argument("--bp-threads", type=int, default=0,
@@ -133,8 +136,8 @@ def main():
             continue
         mode = {"fused": MODE_FUSED, "graph": MODE_GRAPH, "bitplane": MODE_BITPLANE}[form]
         log(f"[bench] {args.design} form={form} lanes={lanes} cycles={cycles}")
-        dt, st, h = run_form(cd, lanes, mode, steps, cps, args.warmup, seed, args.chunk,
-                             args.bp_threads)
+        dt, st, h = run_form(cd, lanes, mode, steps, cps, args.warmup, seed,
+                             args.bp_chunk if form == "bitplane" else args.chunk, args.bp_threads)
         nchk = min(args.check, lanes)
         ref = oracle.netlist_run(registry.table_bytes(args.design), seed, nstim=nchk, cycles=cycles,
                                  threads=16)
@@ -152,19 +155,18 @@ def main():
     if args.learned_cycles and lname in registry.load_index():
         lcd = registry.get(lname)
         lc = args.learned_cycles
-        lsteps = max(1, lc // cps)
         res = {}
         for tag, c in (("heuristic", cd), ("learned", lcd)):
-            log(f"[bench] partition={tag} graph form, {lanes} x {lsteps * cps} cycles, "
+            log(f"[bench] partition={tag} graph form, {lanes} x {lc} cycles, "
                 f"{c.manifest['mtasks']} mtasks")
-            dt, _, h = run_form(c, lanes, MODE_GRAPH, lsteps, cps, args.warmup, seed, args.chunk)
-            res[tag] = (lanes * lsteps * cps / dt, h, c.manifest["mtasks"])
+            dt, _, h = run_form(c, lanes, MODE_GRAPH, 1, lc, args.warmup, seed, args.chunk)
+            res[tag] = (lanes * lc / dt, h, c.manifest["mtasks"])
         same = bool((res["heuristic"][1] == res["learned"][1]).all())
         ref = oracle.netlist_run(registry.table_bytes(args.design), seed, nstim=min(16, lanes),
-                                 cycles=lsteps * cps, threads=16)
+                                 cycles=lc, threads=16)
         exact = same and bool((res["learned"][1][:len(ref)] == ref).all())
         sp = res["learned"][0] / res["heuristic"][0]
-        learned = {"form": "graph", "cycles": lsteps * cps,
+        learned = {"form": "graph", "cycles": lc,
                    "heuristic": res["heuristic"][0], "learned": res["learned"][0],
                    "heuristic_mtasks": res["heuristic"][2], "learned_mtasks": res["learned"][2],
                    "speedup": sp, "bit_exact": exact, "kept": bool(exact and sp > 1.0)}
@@ -182,7 +184,7 @@ def main():
         state_bits = man["bitplane"]["state_bits"]
         b_alg = 2 * ((state_bits + 7) // 8)
         per_launch_ms = st["bitplane_ms"] / max(1, st["bitplane_launches"])
-        cyc_per_launch = min(cps, args.chunk)
+        cyc_per_launch = min(cps, args.bp_chunk)
         bytes_per_launch = b_alg * lanes * cyc_per_launch
         kernel = "rf_bp_kernel"
     elif main_form == "fused":
@@ -204,7 +206,7 @@ def main():
     if main_form == "bitplane" and tp.exists():
         t = json.loads(tp.read_text())
         if (t["lanes"], t["cycles_per_launch"], t["bitplane_threads"]) == \
-                (lanes, min(cps, args.chunk), args.bp_threads or 512):
+                (lanes, min(cps, args.bp_chunk), args.bp_threads or 512):
             traffic = t["read_bytes"] + t["write_bytes"]
     ops = man["ops_per_cycle"]
     valu = stim_cycles * ops / forms[main_form]["seconds"]
@@ -238,7 +240,9 @@ def main():
         "config": {"workload": f"random netlist 20k gates / 2k flops / 64 PI / 64 PO / depth 40, "
                                f"{lanes} stimuli x {cycles} cycles",
                    "design": args.design, "stimuli": lanes, "cycles": cycles,
-                   "cycles_per_step": cps, "form": main_form, "mtasks": man["mtasks"]},
+                   "cycles_per_step": cps,
+                   "cycles_per_launch": min(cps, args.bp_chunk if main_form == "bitplane" else args.chunk),
+                   "form": main_form, "mtasks": man["mtasks"]},
         "forms": {k: {kk: vv for kk, vv in v.items() if kk != "stats"} for k, v in forms.items()},
         "learned_partition": learned,
         "bit_exact_sample": all(v["bit_exact_sample"] for v in forms.values()),
