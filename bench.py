@@ -87,6 +87,8 @@ def main():
                     help="netlist (default, the headline) or one of the other synthetic designs")
     ap.add_argument("--check", type=int, default=64, help="stimuli checked against the oracle")
     ap.add_argument("--cpu-sample", type=int, default=64, help="stimuli in the CPU baseline sample")
+    ap.add_argument("--cpu-sample-netlist", type=int, default=2048,
+                    help="stimuli in the netlist workload's CPU baseline sample (about 13 s on 16 threads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run a plain device-to-device copy of MB megabytes three times "
@@ -213,7 +215,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        ns = args.cpu_sample
+        ns = args.cpu_sample_netlist
         cpu_cycles = cycles
         t0 = time.perf_counter()
         oracle.netlist_run(registry.table_bytes(args.design), seed, nstim=ns, cycles=cpu_cycles,
