@@ -3,22 +3,22 @@
 Workload: the seeded random gate-level netlist (20,000 gates, 2,000 flip-flops,
 64 primary inputs, 64 primary outputs, depth 40), 65,536 stimuli x 2,000
 cycles, 64-bit uniform inputs per cycle generated on the device, per-cycle
-FNV-1a-64 output hash per stimulus. A "step" is ``--cycles-per-step`` clock
-cycles of all stimuli (one kernel launch of the bit-plane and fused forms);
-``--steps`` steps are timed (default 4 x 500 = the full 2,000 cycles). Three
-GPU forms are measured:
+FNV-1a-64 output hash per stimulus.
 
-* bitplane: the design lowered to single-bit logic, 64 stimuli per 64-bit
-  word, every node value in LDS, one kernel per chunk of cycles;
-* fused: one per-lane kernel runs whole cycles (one stimulus per thread);
-* graph: one hipGraph of the per-mtask kernels replayed per cycle (the plain
-  form, one stimulus per thread).
+A step is the whole workload from its start: the post-set-up model state is
+restored from a device-resident checkpoint, the stimuli restart at cycle 0
+(a fresh rf_begin) and all 2,000 cycles run in the bit-plane form (the
+design lowered to single-bit logic, 64 stimuli per 64-bit word, node values
+in LDS, one kernel launch). ``--steps`` steps are timed; every step
+recomputes the same hashes, and the last step's are checked against the CPU
+oracle on stimuli sampled from the first, middle and last workgroups and the
+two sides of the workgroup-slot boundary.
 
-``value`` is the fastest form whose hashes match the CPU oracle on the
-checked sample; every form is reported beside it. Every timed run starts
-from a fresh model. The CPU oracle (16 threads, one stimulus per thread,
-byte-per-gate evaluation) is timed on a bounded sample as
-the reported CPU baseline.
+Beside it, each measured once on a bounded number of cycles that does not
+depend on ``--steps``: the fused per-lane kernel, the plain per-mtask hipGraph
+form, and the learned-vs-heuristic partition in the graph form. The CPU
+oracle is timed on a bounded sample as the reported CPU baseline. Optional
+legs are skipped once ``--budget-s`` seconds have passed.
 """
 from __future__ import annotations
 
@@ -36,28 +36,30 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "oracle"))
 
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK = 256 * 4 * 16 * 2.4e9     # int32 lane-ops/s: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz
+CUS = 256
+CLOCK_HZ = 2.4e9                     # MI355X peak engine clock
+VALU_PEAK = CUS * 4 * 16 * CLOCK_HZ  # int32 lane-ops/s: 256 CUs x 4 SIMD x 16 lanes
+WAVE64 = 64
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(cd, lanes, mode, steps, cps, warmup, seed, chunk, bp_threads=0):
-    if warmup:
-        with cd.model(lanes, fused_chunk=chunk, bitplane_threads=bp_threads) as m:
-            m.begin(seed)
-            for _ in range(warmup):
-                m.run_cycles(cps, mode)
-            m.sync()
+def cpu_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def run_form(cd, lanes, mode, cycles, seed, chunk, bp_threads=0):
+    """One bounded run of ``cycles`` cycles from a fresh model; the model's
+    set-up (initial blocks, time-0 settle) runs before the timed region."""
     with cd.model(lanes, fused_chunk=chunk, bitplane_threads=bp_threads) as m:
         m.begin(seed)
-        m.run_cycles(0, mode)      # model set-up: initial blocks and the time-0 settle
+        m.run_cycles(0, mode)
         m.sync()
         m.reset_stats()
         t0 = time.perf_counter()
-        for _ in range(steps):
-            m.run_cycles(cps, mode)
+        m.run_cycles(cycles, mode)
         m.sync()
         dt = time.perf_counter() - t0
         st = m.stats()
@@ -65,31 +67,66 @@ def run_form(cd, lanes, mode, steps, cps, warmup, seed, chunk, bp_threads=0):
     return dt, st, h
 
 
+def sample_groups(lanes: int, slots: int):
+    """64-lane groups to check: the first, the last of the first round of
+    workgroup slots and the first of the second, the middle and the last."""
+    ng = (lanes + 63) // 64
+    gs = {0, ng // 2, ng - 1}
+    if slots < ng:
+        gs |= {slots - 1, slots}
+    return sorted(g for g in gs if 0 <= g < ng)
+
+
+def check_sample(table, seed, h, lanes, cycles, groups, per_group=64):
+    """Compare the hashes of the sampled groups with the oracle."""
+    import oracle
+    bad = 0
+    n = 0
+    for g in groups:
+        s0 = g * 64
+        k = min(per_group, lanes - s0)
+        ref = oracle.netlist_run(table, seed, nstim=k, cycles=cycles, stim0=s0, threads=cpu_threads())
+        bad += int((h[s0:s0 + k] != ref).sum())
+        n += k
+    return bad == 0, n, bad
+
+
+def profile_json(name: str):
+    p = ROOT / "profiles" / name
+    return json.loads(p.read_text()) if p.exists() else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cycles-per-step", type=int, default=2000,
-                    help="cycles per step (default: one step is the whole 2,000-cycle workload)")
+    ap.add_argument("--cycles", type=int, default=2000, help="cycles per stimulus in one step")
     ap.add_argument("--lanes", type=int, default=65536)
     ap.add_argument("--design", default="net2")
-    ap.add_argument("--forms", default="bitplane,fused,graph")
-    ap.add_argument("--chunk", type=int, default=512, help="cycles per fused launch")
     ap.add_argument("--bp-chunk", type=int, default=2048,
                     help="cycles per bit-plane launch (the state stays in LDS across them)")
-    ap.add_argument("--learned-cycles", type=int, default=400,
-                    help="cycles of the learned-vs-heuristic partition comparison (0: skip)")
     ap.add_argument("--bp-threads", type=int, default=0,
-                    help="bit-plane form: threads per 64-lane workgroup (0: runtime default)")
+                    help="bit-plane form: threads per 64-lane workgroup (0: the autotuned choice)")
+    ap.add_argument("--chunk", type=int, default=512, help="cycles per fused launch")
+    ap.add_argument("--fused-cycles", type=int, default=256,
+                    help="cycles of the fused-form comparison (0: skip)")
+    ap.add_argument("--graph-cycles", type=int, default=64,
+                    help="cycles of the plain graph-form comparison (0: skip)")
+    ap.add_argument("--learned-cycles", type=int, default=64,
+                    help="cycles of the learned-vs-heuristic partition comparison (0: skip)")
     ap.add_argument("--workload", default="netlist",
                     choices=["netlist", "alu", "systolic", "rv32i", "net5s"],
                     help="netlist (default, the headline) or one of the other synthetic designs")
-    ap.add_argument("--check", type=int, default=64, help="stimuli checked against the oracle")
-    ap.add_argument("--cpu-sample", type=int, default=64, help="stimuli in the CPU baseline sample")
-    ap.add_argument("--cpu-sample-netlist", type=int, default=2048,
-                    help="stimuli in the netlist workload's CPU baseline sample (about 13 s on 16 threads)")
+    ap.add_argument("--check", type=int, default=64, help="stimuli checked per sampled group")
+    ap.add_argument("--cpu-sample", type=int, default=64,
+                    help="stimuli in the CPU baseline sample of the other workloads")
+    ap.add_argument("--cpu-sample-netlist", type=int, default=1024,
+                    help="stimuli in the netlist CPU baseline sample (x the full cycles)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--budget-s", type=float, default=300.0,
+                    help="optional legs (other forms, learned partition, CPU baseline) are "
+                         "skipped once this many seconds have passed")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run a plain device-to-device copy of MB megabytes three times "
                          "(the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -115,58 +152,109 @@ def main():
         sys.exit(1)
     if args.workload != "netlist":
         return bench_other(args)
+    return bench_netlist(args, torch)
+
+
+def bench_netlist(args, torch):
     from rtlflow_amd.designs import registry
     from rtlflow_amd.model import MODE_BITPLANE, MODE_FUSED, MODE_GRAPH
     import oracle
 
+    t_start = time.perf_counter()
+
+    def budget_left(leg):
+        if time.perf_counter() - t_start > args.budget_s:
+            log(f"[bench] {leg}: skipped (past the {args.budget_s:.0f} s budget)")
+            return False
+        return True
+
     seed = registry.SEED_CFG[2]
     cd = registry.get(args.design)
+    man = cd.manifest
+    table = registry.table_bytes(args.design)
     if not args.bp_threads:
         from rtlflow_amd.autotune import choice
         tuned = choice(args.design)
-        if tuned and tuned.get("form") == "bitplane":
-            args.bp_threads = tuned["knobs"]["bp_threads"]
-    man = cd.manifest
-    lanes, cps, steps = args.lanes, args.cycles_per_step, args.steps
-    cycles = cps * steps
+        args.bp_threads = (tuned["knobs"]["bp_threads"]
+                           if tuned and tuned.get("form") == "bitplane" else 512)
+    if not man.get("bitplane", {}).get("supported"):
+        raise SystemExit(f"{args.design} has no bit-plane program: {man.get('bitplane')}")
+    lanes, cycles, K, W = args.lanes, args.cycles, args.steps, args.warmup
     stim_cycles = lanes * cycles
-    forms = {}
-    bp_ok = bool(man.get("bitplane", {}).get("supported"))
-    for form in args.forms.split(","):
-        if form == "bitplane" and not bp_ok:
-            log(f"[bench] {args.design}: no bit-plane program ({man.get('bitplane')}); skipped")
-            continue
-        mode = {"fused": MODE_FUSED, "graph": MODE_GRAPH, "bitplane": MODE_BITPLANE}[form]
-        log(f"[bench] {args.design} form={form} lanes={lanes} cycles={cycles}")
-        dt, st, h = run_form(cd, lanes, mode, steps, cps, args.warmup, seed,
-                             args.bp_chunk if form == "bitplane" else args.chunk, args.bp_threads)
-        nchk = min(args.check, lanes)
-        ref = oracle.netlist_run(registry.table_bytes(args.design), seed, nstim=nchk, cycles=cycles,
-                                 threads=16)
-        ok = bool((h[:nchk] == ref).all())
-        log(f"[bench] form={form} {dt:.3f}s {stim_cycles / dt:.4g} stim-cycles/s oracle_match={ok}")
-        forms[form] = {"seconds": dt, "value": stim_cycles / dt, "bit_exact_sample": ok,
-                       "checked_stimuli": nchk, "stats": st}
+    nt = args.bp_threads
+    # workgroup slots of the bit-plane kernel: LDS per group decides how many
+    # groups share a CU
+    lds_per_group = man["bitplane"]["slots"] * 8
+    per_cu = max(1, min(160 * 1024 // max(1, lds_per_group), 2048 // nt))
+    slots = CUS * per_cu
 
-    # learned partition: the plain graph form of the same design partitioned
-    # with the cost model's predicted per-vertex costs, against the
-    # instruction-count heuristic, on the same stimuli; kept only if faster
-    # and bit-exact
+    # ---- set-up (untimed): model, initial blocks + time-0 settle, and a
+    # device-resident checkpoint of that state
+    log(f"[bench] {args.design} bit-plane: {lanes} stimuli x {cycles} cycles per step, "
+        f"{nt} threads per 64-lane group, {K} steps after {W} warm-up")
+    m = cd.model(lanes, fused_chunk=args.bp_chunk, bitplane_threads=nt)
+    m.begin(seed)
+    m.run_cycles(0, MODE_BITPLANE)
+    m.sync()
+    nb = m.checkpoint_bytes()
+    ck = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    m.checkpoint_to(ck.data_ptr(), nb)
+
+    def step():
+        m.restore_from(ck.data_ptr(), nb)
+        m.begin(seed)
+        m.run_cycles(cycles, MODE_BITPLANE)
+
+    for _ in range(W):
+        step()
+    m.sync()
+    m.reset_stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        step()
+    m.sync()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = m.stats()
+    h = m.hashes()
+    m.close()
+    groups = sample_groups(lanes, slots)
+    ok, nchk, bad = check_sample(table, seed, h, lanes, cycles, groups, args.check)
+    value = K * stim_cycles / dt
+    log(f"[bench] bitplane {dt:.3f}s for {K} steps, {value:.4g} stim-cycles/s; oracle sample of "
+        f"{nchk} stimuli in groups {groups}: {'bit-exact' if ok else f'{bad} differ'}")
+    forms = {"bitplane": {"seconds": dt, "steps": K, "value": value, "bit_exact_sample": ok,
+                          "checked_stimuli": nchk, "checked_groups": groups, "cycles": cycles}}
+
+    # ---- the other forms, once each on a bounded number of cycles
+    for form, mode, cyc in (("fused", MODE_FUSED, args.fused_cycles),
+                            ("graph", MODE_GRAPH, args.graph_cycles)):
+        if not cyc or not budget_left(form):
+            continue
+        log(f"[bench] {args.design} form={form} lanes={lanes} cycles={cyc}")
+        fdt, _, fh = run_form(cd, lanes, mode, cyc, seed, args.chunk)
+        fok, fn, _ = check_sample(table, seed, fh, lanes, cyc, [0, (lanes + 63) // 64 - 1], args.check)
+        forms[form] = {"seconds": fdt, "value": lanes * cyc / fdt, "bit_exact_sample": fok,
+                       "checked_stimuli": fn, "cycles": cyc}
+        log(f"[bench] form={form} {fdt:.3f}s {lanes * cyc / fdt:.4g} stim-cycles/s oracle_match={fok}")
+
+    # ---- learned partition: the plain graph form partitioned by the cost
+    # model's choice of granularity against the instruction-count heuristic
     learned = None
     lname = args.design + "_learned"
-    if args.learned_cycles and lname in registry.load_index():
+    if args.learned_cycles and lname in registry.load_index() and budget_left("learned partition"):
         lcd = registry.get(lname)
         lc = args.learned_cycles
         res = {}
         for tag, c in (("heuristic", cd), ("learned", lcd)):
             log(f"[bench] partition={tag} graph form, {lanes} x {lc} cycles, "
                 f"{c.manifest['mtasks']} mtasks")
-            dt, _, h = run_form(c, lanes, MODE_GRAPH, 1, lc, args.warmup, seed, args.chunk)
-            res[tag] = (lanes * lc / dt, h, c.manifest["mtasks"])
+            ldt, _, lh = run_form(c, lanes, MODE_GRAPH, lc, seed, args.chunk)
+            res[tag] = (lanes * lc / ldt, lh, c.manifest["mtasks"])
         same = bool((res["heuristic"][1] == res["learned"][1]).all())
-        ref = oracle.netlist_run(registry.table_bytes(args.design), seed, nstim=min(16, lanes),
-                                 cycles=lc, threads=16)
-        exact = same and bool((res["learned"][1][:len(ref)] == ref).all())
+        lok, _, _ = check_sample(table, seed, res["learned"][1], lanes, lc, [0], 16)
+        exact = same and lok
         sp = res["learned"][0] / res["heuristic"][0]
         learned = {"form": "graph", "cycles": lc,
                    "heuristic": res["heuristic"][0], "learned": res["learned"][0],
@@ -174,85 +262,78 @@ def main():
                    "speedup": sp, "bit_exact": exact, "kept": bool(exact and sp > 1.0)}
         log(f"[bench] learned/heuristic graph throughput {sp:.3f} (bit-exact {exact})")
 
-    # the reported form: the fastest one whose sample is bit-exact
-    exact = {k: v for k, v in forms.items() if v["bit_exact_sample"]}
-    main_form = max(exact or forms, key=lambda k: forms[k]["value"])
-    f = forms[main_form]
-    st = f["stats"]
-    # algorithmic bytes per stimulus-cycle: 2 * S_seq + T_mem + B_in + B_out in
-    # the form's own state layout (inputs are generated and outputs hashed on
-    # the device, so B_in = B_out = 0 in HBM)
-    if main_form == "bitplane":
-        state_bits = man["bitplane"]["state_bits"]
-        b_alg = 2 * ((state_bits + 7) // 8)
-        per_launch_ms = st["bitplane_ms"] / max(1, st["bitplane_launches"])
-        cyc_per_launch = min(cps, args.bp_chunk)
-        bytes_per_launch = b_alg * lanes * cyc_per_launch
-        kernel = "rf_bp_kernel"
-    elif main_form == "fused":
-        b_alg = 2 * man["bytes_seq"]
-        per_launch_ms = st["fused_ms"] / max(1, st["fused_launches"])
-        bytes_per_launch = b_alg * lanes * min(cps, args.chunk)
-        kernel = "rf_k_fused"
-    else:
-        b_alg = 2 * man["bytes_seq"]
-        per_launch_ms = st["graph_ms"] / max(1, st["graph_launches"])
-        bytes_per_launch = b_alg * lanes
-        kernel = "cycle graph"
+    # ---- roofline of the bit-plane kernel: algorithmic bytes per
+    # stimulus-cycle are the bit-packed state read and written once
+    state_bits = man["bitplane"]["state_bits"]
+    b_alg = 2 * ((state_bits + 7) // 8)
+    launches = max(1, st["bitplane_launches"])
+    per_launch_ms = st["bitplane_ms"] / launches
+    cyc_per_launch = min(cycles, args.bp_chunk)
+    bytes_per_launch = b_alg * lanes * cyc_per_launch
     achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
-    # HBM bytes per launch: rocprofv3 FETCH_SIZE / WRITE_SIZE of the same
-    # launch shape, scaled by a plain copy kernel (profiles/); null for other
-    # shapes
+    shape = {"lanes": lanes, "cycles_per_launch": cyc_per_launch, "bitplane_threads": nt}
     traffic = None
-    tp = ROOT / "profiles" / f"{args.design}_bitplane_traffic.json"
-    if main_form == "bitplane" and tp.exists():
-        t = json.loads(tp.read_text())
-        if (t["lanes"], t["cycles_per_launch"], t["bitplane_threads"]) == \
-                (lanes, min(cps, args.bp_chunk), args.bp_threads or 512):
-            traffic = t["read_bytes"] + t["write_bytes"]
-    ops = man["ops_per_cycle"]
-    valu = stim_cycles * ops / forms[main_form]["seconds"]
+    t = profile_json(f"{args.design}_bitplane_traffic.json")
+    if t and all(t.get(k) == v for k, v in shape.items()):
+        traffic = t["read_bytes"] + t["write_bytes"]
+    # measured limits of the same launch shape from the committed SQ counters:
+    # VALU issue (wave instructions over one per CU per clock), LDS bank
+    # conflict cycles per CU-cycle, HBM traffic over peak bandwidth
+    limits = None
+    q = profile_json(f"{args.design}_bitplane_sq.json")
+    if q and all(q.get(k) == v for k, v in shape.items()):
+        ks = q["kernel_ns"] * 1e-9
+        cu_cycles = CUS * CLOCK_HZ * ks
+        limits = {"valu_issue": q["SQ_INSTS_VALU"] / cu_cycles,
+                  "lds_instr_issue": q["SQ_INSTS_LDS"] / cu_cycles,
+                  "lds_bank_conflict": q["SQ_LDS_BANK_CONFLICT"] / cu_cycles,
+                  "hbm_traffic": (traffic / ks / (HBM_PEAK_GBS * 1e9)) if traffic else None,
+                  "source": f"profiles/{args.design}_bitplane_sq.json",
+                  "kernel_ns": q["kernel_ns"]}
+        cand = {k: v for k, v in limits.items() if isinstance(v, float)}
+        limits["highest"] = max(cand, key=cand.get)
+    gates_m = man["bitplane"]["gates"][1]
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and budget_left("CPU baseline"):
         ns = args.cpu_sample_netlist
-        cpu_cycles = cycles
+        th = cpu_threads()
         t0 = time.perf_counter()
-        oracle.netlist_run(registry.table_bytes(args.design), seed, nstim=ns, cycles=cpu_cycles,
-                           threads=16)
+        oracle.netlist_run(table, seed, nstim=ns, cycles=cycles, threads=th)
         cdt = time.perf_counter() - t0
-        cpu = {"value": ns * cpu_cycles / cdt, "unit": "stimulus-cycles/s", "cores": 16,
-               "kind": "port",
-               "sample": f"{ns} stimuli x {cpu_cycles} cycles of the same netlist and protocol "
-                         f"({cdt:.1f} s), C oracle, one stimulus per thread"}
+        cpu = {"value": ns * cycles / cdt, "unit": "stimulus-cycles/s", "cores": th,
+               "kind": "port", "nproc": os.cpu_count(),
+               "sample": f"{ns} stimuli x {cycles} cycles of the same netlist and protocol "
+                         f"({cdt:.1f} s), C oracle, one stimulus per thread on {th} threads"}
 
     line = {
         "metric": "stimulus-cycles/sec, bit-exact vs Verilator CPU; batched+learned vs plain GPU",
-        "value": f["value"],
+        "value": value,
         "unit": "stimulus-cycles/s",
         "n_gpus": 1,
-        "steps": steps,
-        "warmup": args.warmup,
-        "ms_per_step": f["seconds"] / steps * 1e3,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": dt / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u64 bit planes" if main_form == "bitplane" else "u8/u32 integer",
+        "dtype": "u64",
         "data": "synthetic (seeded netlist, on-device SplitMix64 stimulus)",
         "config": {"workload": f"random netlist 20k gates / 2k flops / 64 PI / 64 PO / depth 40, "
                                f"{lanes} stimuli x {cycles} cycles",
                    "design": args.design, "stimuli": lanes, "cycles": cycles,
-                   "cycles_per_step": cps,
-                   "cycles_per_launch": min(cps, args.bp_chunk if main_form == "bitplane" else args.chunk),
-                   "form": main_form, "mtasks": man["mtasks"]},
-        "forms": {k: {kk: vv for kk, vv in v.items() if kk != "stats"} for k, v in forms.items()},
+                   "cycles_per_launch": cyc_per_launch, "form": "bitplane",
+                   "bitplane_threads": nt, "mtasks": man["mtasks"]},
+        "forms": forms,
         "learned_partition": learned,
         "bit_exact_sample": all(v["bit_exact_sample"] for v in forms.values()),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
-                     "alg_bytes_per_stimulus_cycle": b_alg, "avg_launch_ms": per_launch_ms},
-        "valu": {"scalar_equivalent_ops": valu, "peak_ops": VALU_PEAK, "frac": valu / VALU_PEAK,
-                 "ops_per_stimulus_cycle": ops},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"rf_bp_kernel<{nt}>", "alg_bytes_per_stimulus_cycle": b_alg,
+                     "avg_launch_ms": per_launch_ms, "launches": launches, "limits": limits},
+        "valu": {"gate_records_per_cycle": gates_m,
+                 "note": "VALU issue of the bit-plane kernel is roofline.limits.valu_issue "
+                         "(SQ_INSTS_VALU of the same launch shape)"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define RF_DESIGN_ABI 4
+#define RF_DESIGN_ABI 5
 
 /* storage classes (include/rf_heavy.h:11-17: CData/SData/IData/QData) */
 enum { RF_CLS_C = 0, RF_CLS_S = 1, RF_CLS_I = 2, RF_CLS_Q = 3 };
@@ -114,6 +114,10 @@ typedef struct rf_design {
     const void* k_fused;       /* protocol: ncycles full cycles per lane */
     const void* k_persistent;  /* batched: work queue, refill on finish */
     const void* k_reset;       /* zero one lane's state (for refill / reset) */
+    const void* k_settle;      /* graph form: lanes still changing after the
+                                  captured passes finish the settle loop
+                                  (NULL when nchange == 0); rf_args.ncycles =
+                                  passes already run */
     uint32_t nchange;          /* change-detected variables (0: one pass settles) */
     uint32_t clk_port;         /* index into ports, or ~0u */
     /* algorithmic bytes per stimulus-cycle (layout manifest) */

@@ -42,6 +42,9 @@ typedef struct rf_config {
     uint32_t fused_chunk;     /* fused / bit-plane forms: cycles per kernel launch */
     uint32_t bitplane_threads; /* bit-plane form: threads per 64-lane workgroup
                                   (128, 256, 512, 1024; 0 = 512) */
+    uint32_t lane_pad;        /* SoA padding: extra lanes in every slot's row
+                                 beyond n_lanes rounded up to the block
+                                 (a multiple of 64; 0 = none) */
 } rf_config;
 
 typedef struct rf_model rf_model;
@@ -132,6 +135,20 @@ int rf_set_mem_images(rf_model* m, uint32_t mem, uint32_t nstim, uint32_t nelem,
  * the role of the reference's checkN asserter, src/V3EmitC.cpp:4380-4536).
  * *bytes receives the size; dst may be NULL to query it. */
 int rf_lane_state(rf_model* m, uint32_t lane, void* dst, size_t* bytes);
+
+/* Checkpoint / resume of the protocol form: the whole simulation state of
+ * a model (every SoA slot, the lane-major memories, the per-lane change,
+ * done, hash, cycle-counter and stimulus-id arrays, the device flags and
+ * the protocol seed) as one blob, so a long run can stop and continue in
+ * another model of the same design and lane count -- the role of the
+ * reference's --savable model save/restore (src/V3EmitC.cpp:2957-3039).
+ * rf_checkpoint_bytes gives the blob size; dst / src may be host or device
+ * memory. rf_checkpoint_save waits for queued work; rf_checkpoint_load
+ * checks the header (design hash, lanes, layout) and queues its copies on
+ * the model's stream. */
+int rf_checkpoint_bytes(rf_model* m, size_t* bytes);
+int rf_checkpoint_save(rf_model* m, void* dst, size_t bytes);
+int rf_checkpoint_load(rf_model* m, const void* src, size_t bytes);
 
 /* Time every mtask kernel of one eval in isolation (mean of ``reps``
  * back-to-back launches over all lanes, HIP events on the model's stream);

@@ -272,15 +272,25 @@ def class_bytes(w):
     return 1 if w <= 8 else 2 if w <= 16 else 4 if w <= 32 else 8 if w <= 64 else 4 * ((w + 31) // 32)
 
 
-def protocol(design: ir.Design, seed: int, nstim: int, cycles: int, stim0: int = 0) -> List[int]:
-    """Per-stimulus FNV-1a-64 of the outputs under the clock-cycle protocol."""
+def protocol(design: ir.Design, seed: int, nstim: int, cycles: int, stim0: int = 0,
+             images=None, budgets=None) -> List[int]:
+    """Per-stimulus FNV-1a-64 of the outputs under the clock-cycle protocol.
+    ``images``: {memory name: per-stimulus initial contents, indexed by
+    stimulus id}, loaded before the initial blocks run; ``budgets``: cycles
+    per stimulus (indexed from stim0) instead of ``cycles``."""
     data_inputs = [v for v in design.vars if v.kind == "input" and v not in design.clocks]
     outputs = [v for v in design.vars if v.kind == "output"]
+    mems = {v.name: v for v in design.vars if v.is_mem}
     hashes = []
     for s in range(stim0, stim0 + nstim):
         lane = Lane(design)
+        for name, img in (images or {}).items():
+            row = [int(x) for x in img[s]]
+            mem = lane.m[mems[name]]
+            mem[:len(row)] = row[:len(mem)]
         h = 0xcbf29ce484222325
-        for c in range(cycles):
+        ncyc = cycles if budgets is None else int(budgets[s - stim0])
+        for c in range(ncyc):
             for p, var in enumerate(data_inputs):
                 val = 0
                 for j in range((var.width + 63) // 64):

@@ -15,14 +15,18 @@ Knobs:
   - the form (bit-plane, fused, plain graph, batched queue);
   - bit-plane workgroup size (256 / 512 / 1,024 threads per 64 lanes);
   - fused launch length (cycles per launch);
-  - batched form: lanes in the queue.
+  - batched form: lanes in the queue;
+  - the SoA padding (extra lanes in every slot's row, rf_config.lane_pad)
+    of the per-lane forms.
 
 ``build_candidates(name)`` compiles the compile-time candidates on the CPU;
-``tune(name)`` measures every candidate on the GPU on a bounded workload,
-checks each one's hashes against the first candidate's, and writes the
-fastest to ``rtlflow_amd/tuned/<name>.json`` together with the full table.
-The cache records which library build it measured.
-``python -m rtlflow_amd.autotune NAME...`` runs ``tune`` (GPU).
+``tune(name, checker)`` measures every candidate on the GPU on a bounded
+workload, checks each one's hashes on a sample of stimuli against the
+hashes ``checker`` gives for them (an independent model of the design: the
+tests' CPU oracle, ``tests/autotune_check.py``), and writes the fastest
+bit-exact one to ``rtlflow_amd/tuned/<name>.json`` together with the full
+table; a tune in which no candidate matches fails. The cache records which
+library build it measured.
 """
 from __future__ import annotations
 
@@ -49,7 +53,11 @@ COMPILE_SPACE = {
     # 0: every design in the "calls" form (mtask bodies as functions over the
     # lane-minor arrays) instead of all state in one kernel's registers
     "inline_limit": (6000, 0),
+    # V3Partition's --threads: the critical-path limit of mtask merging
+    "threads": (2, 4, 8, 16),
 }
+# SoA padding: extra lanes in every slot's row (rf_config.lane_pad)
+LANE_PAD = (0, 64)
 MEM_LAYOUT = (64, 1 << 30)
 
 
@@ -78,6 +86,10 @@ def compile_candidates(name: str) -> List[Dict]:
                 o = dict(base)
                 o.update(block=block, lane_major_min_depth=lm, inline_limit=il)
                 out.append(o)
+    # the fusion granularity, one knob at a time from the default shape
+    for t in COMPILE_SPACE["threads"]:
+        if t != base.get("threads"):
+            out.append({"threads": t, "block": 256, "lane_major_min_depth": 64, "inline_limit": 6000})
     return out
 
 
@@ -123,6 +135,8 @@ def _measure(cd, name: str, wl: Dict, form: str, knobs: Dict):
         kw["bitplane_threads"] = knobs["bp_threads"]
     if "chunk" in knobs:
         kw["fused_chunk"] = knobs["chunk"]
+    if "lane_pad" in knobs:
+        kw["lane_pad"] = knobs["lane_pad"]
     if wl["kind"] == "queue":
         lanes = knobs["lanes"]
         with cd.model(lanes, **kw) as m:
@@ -153,19 +167,31 @@ def runtime_candidates(name: str, man: Dict, wl: Dict) -> List[tuple]:
     out = []
     if wl["kind"] == "queue":
         for lanes in (wl["nstim"] // 4, wl["nstim"] // 2, wl["nstim"]):
-            out.append(("batched", {"lanes": lanes}))
+            for pad in LANE_PAD:
+                out.append(("batched", {"lanes": lanes, "lane_pad": pad}))
         return out
     if man.get("bitplane", {}).get("supported"):
         for nt in (256, 512, 1024):
             out.append(("bitplane", {"bp_threads": nt}))
     for chunk in (64, 512):
-        out.append(("fused", {"chunk": chunk}))
+        for pad in LANE_PAD:
+            out.append(("fused", {"chunk": chunk, "lane_pad": pad}))
     out.append(("graph", {}))
     return out
 
 
-def tune(name: str, log=print) -> Dict:
-    """GPU: measure every (library, form, knobs) candidate; write the cache."""
+def sample_indices(name: str, wl: Dict) -> np.ndarray:
+    """Stimuli whose hashes are checked: the first and the last 64."""
+    n = wl["nstim"] if wl["kind"] == "queue" else wl["lanes"]
+    k = min(64, n)
+    idx = list(range(k)) + (list(range(n - k, n)) if n > k else [])
+    return np.array(idx)
+
+
+def tune(name: str, checker, log=print) -> Dict:
+    """GPU: measure every (library, form, knobs) candidate; write the cache.
+    ``checker(name, workload, indices)`` returns the expected hashes of the
+    workload's stimuli ``indices``."""
     from . import CompiledDesign
     from .build import BUILD
     from .designs import registry
@@ -180,18 +206,21 @@ def tune(name: str, log=print) -> Dict:
             libs[vname] = CompiledDesign(lp, json.loads((lp.parent / "manifest.json").read_text()))
             copts[vname] = o
     rows = []
-    ref = None
+    idx = sample_indices(name, wl)
+    want = np.asarray(checker(name, wl, idx), dtype=np.uint64)
     for vname, cd in libs.items():
         for form, knobs in runtime_candidates(name, cd.manifest, wl):
             rate, h = _measure(cd, name, wl, form, knobs)
-            exact = ref is None or bool(np.array_equal(h, ref))
-            if ref is None:
-                ref = h
+            exact = bool(np.array_equal(h[idx], want))
             rows.append({"library": vname, "compile": copts[vname], "form": form, "knobs": knobs,
-                         "rate": rate, "bit_exact": exact, "mtasks": cd.manifest["mtasks"]})
+                         "rate": rate, "bit_exact": exact, "mtasks": cd.manifest["mtasks"],
+                         "checked_stimuli": int(len(idx))})
             if log:
                 log(f"{vname:40s} {form:9s} {json.dumps(knobs):24s} {rate:.4g} exact={exact}")
-    best = max((r for r in rows if r["bit_exact"]), key=lambda r: r["rate"])
+    exact_rows = [r for r in rows if r["bit_exact"]]
+    if not exact_rows:
+        raise RuntimeError(f"autotune {name}: no candidate matches the checker on the sampled stimuli")
+    best = max(exact_rows, key=lambda r: r["rate"])
     res = {"design": name, "library": str(libs[name].lib_path.parent.name), "workload": wl,
            "best": best, "table": rows}
     TUNED.mkdir(parents=True, exist_ok=True)
@@ -205,17 +234,3 @@ def choice(name: str) -> Optional[Dict]:
     if not p.exists():
         return None
     return json.loads(p.read_text())["best"]
-
-
-def main(argv=None):
-    import sys
-    names = (argv if argv is not None else sys.argv[1:]) or list(WORKLOADS)
-    for n in names:
-        r = tune(n, log=lambda *a: print(*a, flush=True))
-        b = r["best"]
-        print(f"best {n}: {b['library']} {b['form']} {b['knobs']} {b['rate']:.4g} stimulus-cycles/s",
-              flush=True)
-
-
-if __name__ == "__main__":
-    main()

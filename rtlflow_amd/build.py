@@ -95,6 +95,7 @@ def build_design(name: str, sources: Dict[str, str], manifest: dict, jobs: int =
     for fn, text in sources.items():
         (d / fn).write_text(text)
     (d / "manifest.json").write_text(json.dumps(manifest, indent=1, default=str))
+    emit_makefiles(d, name, [fn for fn in sources if fn.endswith(".hip")])
     objs = []
     from concurrent.futures import ThreadPoolExecutor
 

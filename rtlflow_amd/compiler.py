@@ -783,6 +783,58 @@ def emit_init(comp: Compiled) -> str:
     return "\n".join(lines)
 
 
+def emit_settle(comp: Compiled) -> Optional[str]:
+    """Settle tail of the graph form (designs with change-detected variables
+    only). The cycle graph holds ``settle_unroll`` passes of the mtask DAG;
+    a lane whose change request is still set after them finishes the settle
+    loop here, one thread per lane, pass after pass up to the converge limit.
+    The reference replays the whole mtask graph until change[0] clears, up
+    to 100 times (src/V3EmitC.cpp:5474-5491); lanes are independent and a
+    settled lane's mtasks are skipped (src/V3EmitC.cpp:1654), so finishing
+    each lane on its own gives the same values. ``a.ncycles`` carries the
+    passes already run."""
+    if not comp.sch.change_vars:
+        return None
+    if comp.calls:
+        lim = atomic(comp, "or", "&a.flags[RF_FLAG_UNCONVERGED]", "1u")
+        lines = [kernel_header(comp, "rf_k_settle"),
+                 f"    const uint32_t lane = {lane_expr(comp)};",
+                 "    if (lane >= a.nlanes) return;",
+                 "    if (a.done[lane] || !a.change[lane]) return;",
+                 "    for (uint32_t it = a.ncycles;; ++it) {",
+                 f"        if (it >= a.converge_limit) {{ {lim}; break; }}"]
+        lines += [f"        rf_body_{k}(a, lane);" for k in range(len(comp.mtasks))]
+        lines += ["        if (!rf_body_latch(a, lane)) break;",
+                  f"        if (it + 1u >= a.converge_limit) {{ {lim}; break; }}",
+                  "    }",
+                  "}"]
+        return "\n".join(lines)
+    gen = Gen(comp)
+    gen.top_lines = []
+    sch = comp.sch
+    loc = _vertex_vars(comp, list(range(len(sch.vertices))))
+    loc |= set(comp.design.clocks) | set(comp.clklast.values()) | set(comp.chglast.values()) | \
+        set(sch.change_vars)
+    gen.local = loc
+    plan_packing(comp, gen, comp.stored)
+    # every value the loop touches is in a register before the loop, so its
+    # back edge sees one consistent register state
+    for v in sorted(loc, key=lambda x: x.id):
+        if v in comp.stored:
+            gen.ensure(v)
+        else:
+            gen.declare(v)
+            gen.avail.add(v)
+    _eval_loop_body(comp, gen, start="a.ncycles")
+    for v in sorted(gen.dirty, key=lambda x: x.id):
+        if v in comp.stored:
+            gen.store(v)
+    lines = [kernel_header(comp, "rf_k_settle")]
+    _prologue(comp, gen, lines, check_done=True, check_change=True)
+    lines += gen.top_lines + gen.lines + ["}"]
+    return "\n".join(lines)
+
+
 def clk0_cone(comp: Compiled) -> Optional[List[int]]:
     """The vertices a cycle's clk = 0 eval must run once the design has
     settled: the combinational logic that reads the inputs (new this cycle)
@@ -858,12 +910,16 @@ def _eval_loop_body_split(comp: Compiled, gen: Gen, first: str, cone: Optional[L
     gen.emit("}")
 
 
-def _eval_loop_body(comp: Compiled, gen: Gen):
-    """One eval() with the settle loop on device (src/V3EmitC.cpp:5426-5496)."""
+def _eval_loop_body(comp: Compiled, gen: Gen, start: str = "0u"):
+    """One eval() with the settle loop on device (src/V3EmitC.cpp:5426-5496).
+    ``start``: settle passes the eval has already run (the graph form's
+    unrolled passes, for the settle-tail kernel)."""
     all_vids = list(comp.sch.topo)
     if comp.sch.change_vars:
-        gen.emit("for (uint32_t it = 0u;; ++it) {")
+        gen.emit(f"for (uint32_t it = {start};; ++it) {{")
         gen.ind += 1
+        if start != "0u":
+            gen.emit(f"if (it >= a.converge_limit) {{ {atomic(comp, 'or', '&a.flags[RF_FLAG_UNCONVERGED]', '1u')}; break; }}")
     _emit_vertices(comp, gen, all_vids, store_in_guard=False)
     for clk, last in comp.clklast.items():
         gen.emit(gen.set(last, gen.get(clk)))
@@ -1169,6 +1225,10 @@ def _emit_source(comp: Compiled) -> str:
         from .persistent import emit_persistent
         parts.append(emit_persistent(comp))
         parts.append("")
+    tail = emit_settle(comp)
+    if tail:
+        parts.append(tail)
+        parts.append("")
     body = "\n".join(parts)
     h = int(hashlib.sha1(body.encode()).hexdigest()[:16], 16)
     parts = [body, emit_descriptor(comp, h)]
@@ -1231,6 +1291,7 @@ def emit_descriptor(comp: Compiled, h: int) -> str:
     out.append(f"    {len(ports)}u, rf_ports, {len(mems)}u, rf_mems, {len(comp.mtasks)}u, rf_mtasks,")
     out.append("    (const void*)&rf_k_init, (const void*)&rf_k_eval_end, (const void*)&rf_k_stim,")
     out.append("    (const void*)&rf_k_fused, (const void*)&rf_k_persistent, (const void*)&rf_k_reset,")
+    out.append("    (const void*)&rf_k_settle," if comp.sch.change_vars else "    0,")
     out.append(f"    {len(comp.sch.change_vars)}u, {clk_idx}u,")
     out.append(f"    {man['bytes_seq']}ull, 0ull, {man['bytes_in']}ull, {man['bytes_out']}ull,")
     out.append(f"    {man['ops_per_cycle']}ull,")

@@ -86,6 +86,30 @@ int eval(HostModel& m, const std::vector<uint32_t>& order, uint32_t final_phase)
     return 0;
 }
 
+// One eval as the device's cycle graph runs it: ``unroll`` passes of the
+// mtask DAG (each masked by the lanes' change requests and closed by an
+// eval-end), then the settle-tail kernel for lanes still changing, then the
+// eval-end that does the phase work.
+int eval_graph_form(HostModel& m, const std::vector<uint32_t>& order, uint32_t final_phase,
+                    uint32_t unroll) {
+    const rf_design* d = m.d;
+    if (!d->nchange || !d->k_settle) {
+        for (uint32_t k : order) m.all(d->mtasks[k].kernel, RF_PHASE_EVAL);
+        m.all(d->k_eval_end, final_phase);
+        return 0;
+    }
+    for (uint32_t it = 0; it < unroll; ++it) {
+        for (uint32_t k : order) m.all(d->mtasks[k].kernel, RF_PHASE_EVAL);
+        m.all(d->k_eval_end, RF_PHASE_EVAL);
+    }
+    rf_args x = m.a;
+    x.ncycles = unroll;
+    host_kernel f = (host_kernel)d->k_settle;
+    for (uint32_t l = 0; l < m.n; ++l) f(x, l);
+    m.all(d->k_eval_end, final_phase);
+    return 0;
+}
+
 std::vector<uint32_t> topo_order(const rf_design* d, std::mt19937_64* rng) {
     const uint32_t n = d->nmtasks;
     std::vector<uint32_t> indeg(n, 0);
@@ -144,7 +168,9 @@ static void load_images(HostModel& m, const uint64_t* images, uint32_t stim0) {
 }
 
 // mode 0: mtasks in table order; 1: fused kernel; 2: random topological order
-// per eval (seeded by order_seed). images: optional per-stimulus memory images.
+// per eval (seeded by order_seed); 3: the device cycle graph's schedule with
+// two captured settle passes and the settle-tail kernel. images: optional
+// per-stimulus memory images.
 RF_EXPORT int rf_host_protocol(const rf_design* d, uint32_t nlanes, uint64_t seed, uint32_t stim0,
                                uint32_t ncycles, uint32_t mode, uint64_t order_seed, uint64_t* hashes,
                                const uint64_t* images) {
@@ -163,9 +189,11 @@ RF_EXPORT int rf_host_protocol(const rf_design* d, uint32_t nlanes, uint64_t see
         for (uint32_t c = 0; c < ncycles; ++c) {
             m.all(d->k_stim, RF_PHASE_EVAL);
             auto o1 = topo_order(d, mode == 2 ? &rng : nullptr);
-            if (eval(m, o1, RF_PHASE_RISE)) return -4;
+            if (mode == 3 ? eval_graph_form(m, o1, RF_PHASE_RISE, 2) : eval(m, o1, RF_PHASE_RISE))
+                return -4;
             auto o2 = topo_order(d, mode == 2 ? &rng : nullptr);
-            if (eval(m, o2, RF_PHASE_HASH)) return -4;
+            if (mode == 3 ? eval_graph_form(m, o2, RF_PHASE_HASH, 2) : eval(m, o2, RF_PHASE_HASH))
+                return -4;
         }
     }
     if (m.flags[RF_FLAG_UNCONVERGED]) return -4;

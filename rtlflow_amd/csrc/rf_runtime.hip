@@ -73,6 +73,7 @@ struct rf_model {
     uint8_t* images = nullptr;     // device table of per-memory image descriptors
     std::vector<void*> image_bufs;
     bool did_init = false;
+    bool graphs_stale = false;     // the captured graphs hold an old seed
     hipGraphExec_t eval_exec = nullptr;
     hipGraph_t eval_graph = nullptr;
     hipGraphExec_t cycle_exec = nullptr;
@@ -108,11 +109,29 @@ int launch(rf_model* m, const void* fn, const rf_args& a) {
     return RF_OK;
 }
 
+// Add one kernel node over all lanes after ``deps``.
+int add_node(rf_model* m, hipGraph_t g, const void* fn, const rf_args& a, const hipGraphNode_t* deps,
+             size_t ndeps, std::deque<rf_args>& argstore, hipGraphNode_t* node) {
+    hipKernelNodeParams p{};
+    p.func = const_cast<void*>(fn);
+    p.gridDim = dim3(m->grid);
+    p.blockDim = dim3(m->d->block);
+    argstore.push_back(a);
+    void* kp[1] = {&argstore.back()};
+    p.kernelParams = kp;
+    RF_HIP(hipGraphAddKernelNode(node, g, deps, ndeps, &p));
+    return RF_OK;
+}
+
 // Append one eval (mtask DAG, settle_unroll times, each followed by the
 // eval-end node) to a graph. deps: nodes the eval depends on. On return
 // *tail is the last eval-end node. The final eval-end uses ``last_phase``.
+// With ``settle_tail`` (designs with change-detected variables) the passes
+// all end in RF_PHASE_EVAL, then the design's settle-tail kernel runs the
+// lanes still changing to convergence or the converge limit, and one more
+// eval-end node does the phase work.
 int add_eval(rf_model* m, hipGraph_t g, hipGraphNode_t dep, uint32_t unroll, uint32_t last_phase,
-             std::deque<rf_args>& argstore, hipGraphNode_t* tail) {
+             bool settle_tail, std::deque<rf_args>& argstore, hipGraphNode_t* tail) {
     const rf_design* d = m->d;
     hipGraphNode_t prev = dep;
     for (uint32_t it = 0; it < unroll; ++it) {
@@ -146,11 +165,21 @@ int add_eval(rf_model* m, hipGraph_t g, hipGraphNode_t dep, uint32_t unroll, uin
         p.func = const_cast<void*>(d->k_eval_end);
         p.gridDim = dim3(m->grid);
         p.blockDim = dim3(d->block);
-        argstore.push_back(args_for(m, it + 1 == unroll ? last_phase : RF_PHASE_EVAL));
+        argstore.push_back(args_for(m, it + 1 == unroll && !settle_tail ? last_phase : RF_PHASE_EVAL));
         void* kp[1] = {&argstore.back()};
         p.kernelParams = kp;
         hipGraphNode_t end;
         RF_HIP(hipGraphAddKernelNode(&end, g, sinks.data(), sinks.size(), &p));
+        prev = end;
+    }
+    if (settle_tail) {
+        rf_args a = args_for(m, RF_PHASE_EVAL);
+        a.ncycles = unroll;
+        hipGraphNode_t st, end;
+        int rc = add_node(m, g, d->k_settle, a, &prev, 1, argstore, &st);
+        if (rc) return rc;
+        rc = add_node(m, g, d->k_eval_end, args_for(m, last_phase), &st, 1, argstore, &end);
+        if (rc) return rc;
         prev = end;
     }
     *tail = prev;
@@ -164,7 +193,7 @@ int build_graphs(rf_model* m) {
     // eval graph: one eval (used by rf_eval, one settle iteration per replay)
     RF_HIP(hipGraphCreate(&m->eval_graph, 0));
     hipGraphNode_t tail;
-    int rc = add_eval(m, m->eval_graph, nullptr, 1, RF_PHASE_EVAL, argstore, &tail);
+    int rc = add_eval(m, m->eval_graph, nullptr, 1, RF_PHASE_EVAL, false, argstore, &tail);
     if (rc) return rc;
     RF_HIP(hipGraphInstantiate(&m->eval_exec, m->eval_graph, nullptr, nullptr, 0));
     // cycle graph: stim -> eval(clk=0) -> rise -> eval(clk=1) -> hash
@@ -179,10 +208,11 @@ int build_graphs(rf_model* m) {
     hipGraphNode_t stim;
     RF_HIP(hipGraphAddKernelNode(&stim, m->cycle_graph, nullptr, 0, &p));
     const uint32_t unroll = d->nchange ? std::max<uint32_t>(1, m->cfg.settle_unroll) : 1;
+    const bool settle_tail = d->nchange && d->k_settle;
     hipGraphNode_t t1, t2;
-    rc = add_eval(m, m->cycle_graph, stim, unroll, RF_PHASE_RISE, argstore, &t1);
+    rc = add_eval(m, m->cycle_graph, stim, unroll, RF_PHASE_RISE, settle_tail, argstore, &t1);
     if (rc) return rc;
-    rc = add_eval(m, m->cycle_graph, t1, unroll, RF_PHASE_HASH, argstore, &t2);
+    rc = add_eval(m, m->cycle_graph, t1, unroll, RF_PHASE_HASH, settle_tail, argstore, &t2);
     if (rc) return rc;
     RF_HIP(hipGraphInstantiate(&m->cycle_exec, m->cycle_graph, nullptr, nullptr, 0));
     return RF_OK;
@@ -269,6 +299,38 @@ int timed_end(rf_model* m, hipEvent_t e0, uint32_t kind, uint32_t count) {
 
 int check_flags(rf_model* m);
 
+// rf_begin's per-lane protocol state, set on the device so that a begin
+// queues behind earlier launches without a host wait.
+__global__ void rf_rt_begin(uint64_t* hash, uint32_t* stim, uint32_t* cyc, uint64_t npad,
+                            uint32_t stim0) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npad) return;
+    hash[k] = 0xcbf29ce484222325ull;  // FNV-1a-64 offset basis
+    stim[k] = stim0 + (uint32_t)k;
+    cyc[k] = 0u;
+}
+
+void drop_graphs(rf_model* m) {
+    if (m->cycle_exec) (void)hipGraphExecDestroy(m->cycle_exec);
+    if (m->cycle_graph) (void)hipGraphDestroy(m->cycle_graph);
+    if (m->eval_exec) (void)hipGraphExecDestroy(m->eval_exec);
+    if (m->eval_graph) (void)hipGraphDestroy(m->eval_graph);
+    m->cycle_exec = nullptr;
+    m->cycle_graph = nullptr;
+    m->eval_exec = nullptr;
+    m->eval_graph = nullptr;
+}
+
+// Rebuild the graphs when their kernel arguments are out of date.
+int fresh_graphs(rf_model* m) {
+    if (!m->graphs_stale && m->cycle_exec) return RF_OK;
+    drop_graphs(m);
+    int rc = build_graphs(m);
+    if (rc) return rc;
+    m->graphs_stale = false;
+    return RF_OK;
+}
+
 // Wait for every queued launch, fold the HIP-event times into the
 // statistics and (``check``) report the run's device flags.
 int settle(rf_model* m, bool check = true) {
@@ -334,6 +396,8 @@ int rf_create(const rf_design* d, uint32_t n_lanes, const rf_config* cfg, rf_mod
         !(cfg->bitplane_threads == 128 || cfg->bitplane_threads == 256 || cfg->bitplane_threads == 512 ||
           cfg->bitplane_threads == 1024))
         return fail(RF_E_ARG, "bitplane_threads must be 128, 256, 512 or 1024 (got %u)", cfg->bitplane_threads);
+    if (cfg && cfg->lane_pad % 64)
+        return fail(RF_E_ARG, "lane_pad must be a multiple of 64 (got %u)", cfg->lane_pad);
     rf_model* m = new rf_model();
     m->d = d;
     m->cfg.settle_unroll = 2;
@@ -345,10 +409,11 @@ int rf_create(const rf_design* d, uint32_t n_lanes, const rf_config* cfg, rf_mod
         if (cfg->converge_limit) m->cfg.converge_limit = cfg->converge_limit;
         if (cfg->fused_chunk) m->cfg.fused_chunk = cfg->fused_chunk;
         if (cfg->bitplane_threads) m->cfg.bitplane_threads = cfg->bitplane_threads;
+        m->cfg.lane_pad = cfg->lane_pad;
     }
     m->nlanes = n_lanes;
     const uint64_t q = d->block > 64 ? d->block : 64;
-    m->npad = ((uint64_t)n_lanes + q - 1) / q * q;
+    m->npad = ((uint64_t)n_lanes + q - 1) / q * q + m->cfg.lane_pad;
     m->grid = (uint32_t)((n_lanes + d->block - 1) / d->block);
     auto cleanup = [&](int rc) {
         rf_destroy(m);
@@ -437,10 +502,7 @@ int rf_destroy(rf_model* m) {
         (void)hipEventDestroy(q.e1);
     }
     for (hipEvent_t e : m->evfree) (void)hipEventDestroy(e);
-    if (m->cycle_exec) (void)hipGraphExecDestroy(m->cycle_exec);
-    if (m->cycle_graph) (void)hipGraphDestroy(m->cycle_graph);
-    if (m->eval_exec) (void)hipGraphExecDestroy(m->eval_exec);
-    if (m->eval_graph) (void)hipGraphDestroy(m->eval_graph);
+    drop_graphs(m);
     for (int c = 0; c < 4; ++c)
         if (m->cls[c]) (void)hipFree(m->cls[c]);
     void* ps[] = {m->mem, m->change, m->done, m->hash, m->cyc, m->stim, m->flags,
@@ -579,6 +641,8 @@ int rf_eval(rf_model* m) {
     }
     int rc = do_init(m);
     if (rc) return rc;
+    rc = fresh_graphs(m);
+    if (rc) return rc;
     const rf_design* d = m->d;
     if (!d->nchange) {
         RF_HIP(hipGraphLaunch(m->eval_exec, m->stream));
@@ -613,26 +677,14 @@ int rf_begin(rf_model* m, uint64_t seed, uint32_t stim0) {
         const int src = settle(m);
         if (src) return src;
     }
-    std::vector<uint64_t> h(m->npad, 0xcbf29ce484222325ull);  // FNV-1a-64 offset basis
-    std::vector<uint32_t> s(m->npad);
-    for (uint64_t k = 0; k < m->npad; ++k) s[k] = stim0 + (uint32_t)k;
-    RF_HIP(hipMemcpyAsync(m->hash, h.data(), m->npad * 8, hipMemcpyHostToDevice, m->stream));
-    RF_HIP(hipMemcpyAsync(m->stim, s.data(), m->npad * 4, hipMemcpyHostToDevice, m->stream));
-    RF_HIP(hipMemsetAsync(m->cyc, 0, m->npad * 4, m->stream));
+    const uint32_t tpb = 256;
+    hipLaunchKernelGGL(rf_rt_begin, dim3((uint32_t)((m->npad + tpb - 1) / tpb)), dim3(tpb), 0, m->stream,
+                       m->hash, m->stim, m->cyc, m->npad, stim0);
+    RF_HIP(hipGetLastError());
     RF_HIP(hipMemsetAsync(m->flags, 0, RF_FLAG_WORDS * 4, m->stream));
-    RF_HIP(hipStreamSynchronize(m->stream));
+    // the graphs captured the old seed: rebuilt before their next replay
+    if (seed != m->base.seed) m->graphs_stale = true;
     m->base.seed = seed;
-    // graphs captured the old seed: rebuild their kernel arguments
-    if (m->cycle_exec) {
-        (void)hipGraphExecDestroy(m->cycle_exec);
-        (void)hipGraphDestroy(m->cycle_graph);
-        (void)hipGraphExecDestroy(m->eval_exec);
-        (void)hipGraphDestroy(m->eval_graph);
-        m->cycle_exec = nullptr;
-        m->eval_exec = nullptr;
-        int rc = build_graphs(m);
-        if (rc) return rc;
-    }
     return RF_OK;
 }
 
@@ -648,6 +700,8 @@ int rf_run_cycles(rf_model* m, uint32_t n, uint32_t mode) {
     hipEvent_t e0;
     if (mode == RF_MODE_GRAPH) {
         if (!n) return RF_OK;
+        rc = fresh_graphs(m);
+        if (rc) return rc;
         rc = timed_begin(m, &e0);
         if (rc) return rc;
         for (uint32_t c = 0; c < n; ++c) RF_HIP(hipGraphLaunch(m->cycle_exec, m->stream));
@@ -752,6 +806,37 @@ static int queue_setup(rf_model* m, const uint32_t* budgets, uint32_t nstim, con
     return RF_OK;
 }
 
+// The persistent launch of a batched workload over device queue buffers
+// (budget, order, out_hash), the host order ``ord`` already checked.
+static int run_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
+                     const uint32_t* ord, uint32_t* dbudget, uint32_t* dorder, uint64_t* dout,
+                     uint64_t* out_hashes) {
+    RF_HIP(hipMemcpyAsync(dbudget, budgets, (size_t)nstim * 4, hipMemcpyHostToDevice, m->stream));
+    RF_HIP(hipMemcpyAsync(dorder, ord, (size_t)nstim * 4, hipMemcpyHostToDevice, m->stream));
+    uint32_t f[RF_FLAG_WORDS] = {0};
+    f[RF_FLAG_QUEUE] = m->nlanes;
+    RF_HIP(hipMemcpyAsync(m->flags, f, sizeof f, hipMemcpyHostToDevice, m->stream));
+    rf_args a = args_for(m, RF_PHASE_EVAL);
+    a.seed = seed;
+    a.budget = dbudget;
+    a.order = dorder;
+    a.out_hash = dout;
+    a.nstim = nstim;
+    RF_HIP(hipEventRecord(m->ev0, m->stream));
+    int rc = launch(m, m->d->k_persistent, a);
+    if (rc) return rc;
+    RF_HIP(hipEventRecord(m->ev1, m->stream));
+    RF_HIP(hipEventSynchronize(m->ev1));
+    float ms = 0;
+    RF_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    m->stats.fused_launches++;
+    m->stats.fused_ms += ms;
+    RF_HIP(hipMemcpyAsync(out_hashes, dout, (size_t)nstim * 8, hipMemcpyDeviceToHost, m->stream));
+    RF_HIP(hipStreamSynchronize(m->stream));
+    m->did_init = true;
+    return check_flags(m);
+}
+
 int rf_submit_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
                     const uint32_t* order, uint64_t* out_hashes) {
     if (!m || !budgets || !out_hashes) return fail(RF_E_ARG, "null argument");
@@ -763,31 +848,7 @@ int rf_submit_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_
     std::vector<uint32_t> ord;
     int qrc = queue_setup(m, budgets, nstim, order, ord);
     if (qrc) return qrc;
-    RF_HIP(hipMemcpyAsync(m->budget, budgets, (size_t)nstim * 4, hipMemcpyHostToDevice, m->stream));
-    RF_HIP(hipMemcpyAsync(m->order, ord.data(), (size_t)nstim * 4, hipMemcpyHostToDevice, m->stream));
-    RF_HIP(hipMemsetAsync(m->out_hash, 0, (size_t)nstim * 8, m->stream));
-    uint32_t f[RF_FLAG_WORDS] = {0};
-    f[RF_FLAG_QUEUE] = m->nlanes;
-    RF_HIP(hipMemcpyAsync(m->flags, f, sizeof f, hipMemcpyHostToDevice, m->stream));
-    rf_args a = args_for(m, RF_PHASE_EVAL);
-    a.seed = seed;
-    a.budget = m->budget;
-    a.order = m->order;
-    a.out_hash = m->out_hash;
-    a.nstim = nstim;
-    RF_HIP(hipEventRecord(m->ev0, m->stream));
-    int rc = launch(m, m->d->k_persistent, a);
-    if (rc) return rc;
-    RF_HIP(hipEventRecord(m->ev1, m->stream));
-    RF_HIP(hipEventSynchronize(m->ev1));
-    float ms = 0;
-    RF_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
-    m->stats.fused_launches++;
-    m->stats.fused_ms += ms;
-    RF_HIP(hipMemcpyAsync(out_hashes, m->out_hash, (size_t)nstim * 8, hipMemcpyDeviceToHost, m->stream));
-    RF_HIP(hipStreamSynchronize(m->stream));
-    m->did_init = true;
-    return check_flags(m);
+    return run_queue(m, seed, budgets, nstim, ord.data(), m->budget, m->order, m->out_hash, out_hashes);
 }
 
 int rf_submit_queue_bitplane(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
@@ -856,6 +917,132 @@ int rf_submit_queue_bitplane(rf_model* m, uint64_t seed, const uint32_t* budgets
     RF_HIP(hipMemcpyAsync(m->done, done.data(), m->npad, hipMemcpyHostToDevice, m->stream));
     RF_HIP(hipStreamSynchronize(m->stream));
     for (uint32_t k = 0; k < nstim; ++k) out_hashes[k] = budgets[k] ? oh[k] : 0xcbf29ce484222325ull;
+    return RF_OK;
+}
+
+// ---------------------------------------------------------------- checkpoints
+//
+// A checkpoint is one byte blob: a header naming the design and the layout,
+// then every device array of the model in a fixed order. Both ends of a copy
+// may be host or device memory (hipMemcpyDefault).
+
+}  // extern "C"
+
+namespace {
+
+constexpr char kCkptMagic[8] = {'R', 'F', 'C', 'K', 'P', 'T', 0, 1};
+constexpr uint32_t kCkptVersion = 1;
+
+struct ckpt_header {
+    char magic[8];
+    uint32_t version, header_bytes;
+    uint64_t design_hash;
+    uint32_t nlanes, did_init;
+    uint64_t npad;
+    uint64_t slots[4];
+    uint64_t mem_row_bytes;
+    uint64_t seed;
+    uint64_t total_bytes;
+    uint64_t reserved[4];
+};
+static_assert(sizeof(ckpt_header) == 128, "checkpoint header is 128 bytes");
+
+struct ckpt_section {
+    void* dev;
+    size_t bytes;
+};
+
+std::vector<ckpt_section> ckpt_sections(const rf_model* m) {
+    std::vector<ckpt_section> out;
+    for (int c = 0; c < 4; ++c) out.push_back({m->cls[c], m->d->slots[c] * m->npad * kClassBytes[c]});
+    out.push_back({m->mem, m->d->mem_row_bytes * m->npad});
+    out.push_back({m->change, m->npad * 4});
+    out.push_back({m->done, m->npad});
+    out.push_back({m->hash, m->npad * 8});
+    out.push_back({m->cyc, m->npad * 4});
+    out.push_back({m->stim, m->npad * 4});
+    out.push_back({m->flags, RF_FLAG_WORDS * 4});
+    return out;
+}
+
+size_t ckpt_total(const rf_model* m) {
+    size_t n = sizeof(ckpt_header);
+    for (const auto& sct : ckpt_sections(m)) n += sct.bytes;
+    return n;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rf_checkpoint_bytes(rf_model* m, size_t* bytes) {
+    if (!m || !bytes) return fail(RF_E_ARG, "null argument");
+    *bytes = ckpt_total(m);
+    return RF_OK;
+}
+
+int rf_checkpoint_save(rf_model* m, void* dst, size_t bytes) {
+    if (!m || !dst) return fail(RF_E_ARG, "null model or buffer");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
+    const size_t total = ckpt_total(m);
+    if (bytes < total) return fail(RF_E_ARG, "checkpoint buffer too small (%zu < %zu)", bytes, total);
+    ckpt_header h{};
+    memcpy(h.magic, kCkptMagic, 8);
+    h.version = kCkptVersion;
+    h.header_bytes = sizeof h;
+    h.design_hash = m->d->hash;
+    h.nlanes = m->nlanes;
+    h.did_init = m->did_init ? 1u : 0u;
+    h.npad = m->npad;
+    for (int c = 0; c < 4; ++c) h.slots[c] = m->d->slots[c];
+    h.mem_row_bytes = m->d->mem_row_bytes;
+    h.seed = m->base.seed;
+    h.total_bytes = total;
+    uint8_t* o = (uint8_t*)dst;
+    RF_HIP(hipMemcpyAsync(o, &h, sizeof h, hipMemcpyDefault, m->stream));
+    o += sizeof h;
+    for (const auto& sct : ckpt_sections(m)) {
+        if (sct.bytes) RF_HIP(hipMemcpyAsync(o, sct.dev, sct.bytes, hipMemcpyDefault, m->stream));
+        o += sct.bytes;
+    }
+    RF_HIP(hipStreamSynchronize(m->stream));
+    return RF_OK;
+}
+
+int rf_checkpoint_load(rf_model* m, const void* src, size_t bytes) {
+    if (!m || !src) return fail(RF_E_ARG, "null model or buffer");
+    {
+        const int rc = settle(m);
+        if (rc) return rc;
+    }
+    ckpt_header h{};
+    if (bytes < sizeof h) return fail(RF_E_ARG, "checkpoint shorter than its header");
+    RF_HIP(hipMemcpyAsync(&h, src, sizeof h, hipMemcpyDefault, m->stream));
+    RF_HIP(hipStreamSynchronize(m->stream));
+    if (memcmp(h.magic, kCkptMagic, 8) || h.version != kCkptVersion || h.header_bytes != sizeof h)
+        return fail(RF_E_ARG, "not a checkpoint of this runtime (magic or version)");
+    if (h.design_hash != m->d->hash)
+        return fail(RF_E_DESIGN, "checkpoint of another design (hash %llx, model %llx)",
+                    (unsigned long long)h.design_hash, (unsigned long long)m->d->hash);
+    if (h.nlanes != m->nlanes || h.npad != m->npad)
+        return fail(RF_E_ARG, "checkpoint of %u lanes, model has %u", h.nlanes, m->nlanes);
+    for (int c = 0; c < 4; ++c)
+        if (h.slots[c] != m->d->slots[c]) return fail(RF_E_DESIGN, "checkpoint layout differs");
+    if (h.mem_row_bytes != m->d->mem_row_bytes) return fail(RF_E_DESIGN, "checkpoint layout differs");
+    const size_t total = ckpt_total(m);
+    if (h.total_bytes != total || bytes < total)
+        return fail(RF_E_ARG, "checkpoint size %zu, expected %zu", bytes, total);
+    const uint8_t* i = (const uint8_t*)src + sizeof h;
+    for (const auto& sct : ckpt_sections(m)) {
+        if (sct.bytes) RF_HIP(hipMemcpyAsync(sct.dev, i, sct.bytes, hipMemcpyDefault, m->stream));
+        i += sct.bytes;
+    }
+    m->did_init = h.did_init != 0;
+    if (h.seed != m->base.seed) m->graphs_stale = true;
+    m->base.seed = h.seed;
     return RF_OK;
 }
 

@@ -258,8 +258,21 @@ def _settle(which: str):
 
 register("circ", _settle("circ"), threads=2)
 register("osc", _settle("osc"), threads=2)
+register("smear", _settle("smear"), threads=2)
 register("gated", _settle("gated"), threads=2)
 
 # random designs inside the bit-plane subset (no memories, no division)
 for _s in (1, 2, 3):
     register(f"bfuzz{_s}", _bfuzz(_s), threads=2)
+
+
+# known-answer designs over the reference's arithmetic test vectors
+def _kat(family: str):
+    def make():
+        from . import kat
+        return {"verilog": kat.verilog(family), "top": f"kat_{family}"}
+    return make
+
+
+for _f in ("shift", "div", "signed", "red"):
+    register(f"kat_{_f}", _kat(_f), threads=2)

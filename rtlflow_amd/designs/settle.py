@@ -6,6 +6,7 @@ Combinational cycles through two always blocks (Verilator's UNOPTFLAT case):
   only the lower nibble of ``y`` (always 0), so after the clk = 0 settle
   ``x = a`` and ``y = a & 0xf0`` whatever the evaluation order, and the
   registered output is ``q = a & 0x0f`` of the previous cycle (closed form).
+* ``smear`` converges after up to 17 passes (see below).
 * ``osc`` never converges (``x = ~y``, ``y = x``): every eval must report
   non-convergence after the settle limit.
 """
@@ -30,6 +31,59 @@ module osc(input clk, input a, output o);
     assign o = q;
 endmodule
 """
+
+# ``smear`` needs many settle passes: ``x = (1 << a) | (y << 1)`` with
+# ``y = x`` through two always blocks is a combinational cycle whose unique
+# fixed point is ``x = 0xffff << a`` (bit i of x is set iff some bit j <= i
+# of ``1 << a`` is). Each pass moves a change one bit further, so settling
+# from the previous cycle's fixed point (shift p) to this cycle's (shift q)
+# takes about q - p + 1 passes and the first cycle from reset up to 17: more
+# than the graph form's captured passes, fewer than the converge limit. The
+# register ``r`` sums the settled x once per cycle.
+SMEAR = r"""
+module smear(input clk, input [3:0] a, output [15:0] o, output [15:0] ox);
+    reg [15:0] x, y, r;
+    always @* x = (16'h0001 << a) | (y << 1);
+    always @* y = x;
+    always @(posedge clk) r <= r + x;
+    assign o = r;
+    assign ox = x;
+endmodule
+"""
+
+
+def smear_closed_form(stim, seed: int, sid: int, cycles: int) -> list:
+    """Per-cycle output bytes (o then ox, 16 bits each, little-endian) of
+    ``smear`` for stimulus ``sid``."""
+    r = 0
+    out = []
+    for cy in range(cycles):
+        a = stim(seed, sid, cy, 0) & 15
+        x = (0xFFFF << a) & 0xFFFF
+        r = (r + x) & 0xFFFF
+        out.append(r.to_bytes(2, "little") + x.to_bytes(2, "little"))
+    return out
+
+
+def smear_passes(stim, seed: int, sid: int, cycles: int) -> list:
+    """Settle passes of the clk = 0 eval in each cycle of ``smear``, counted
+    by iterating the two always blocks from the previous fixed point until
+    nothing changes (the pass that finds no change included)."""
+    x = y = 0
+    out = []
+    for cy in range(cycles):
+        a = stim(seed, sid, cy, 0) & 15
+        n = 0
+        while True:
+            n += 1
+            nx = ((1 << a) | (y << 1)) & 0xFFFF
+            ny = nx
+            if (nx, ny) == (x, y):
+                break
+            x, y = nx, ny
+        out.append(n)
+    return out
+
 
 # A gated clock: ``gclk = clk & en`` (en changes with the other inputs while
 # clk is low), so ``g`` adds ``d`` exactly in the cycles whose ``en`` is set;

@@ -32,7 +32,8 @@ class RFError(RuntimeError):
 
 class rf_config(C.Structure):
     _fields_ = [("settle_unroll", C.c_uint32), ("converge_limit", C.c_uint32),
-                ("fused_chunk", C.c_uint32), ("bitplane_threads", C.c_uint32)]
+                ("fused_chunk", C.c_uint32), ("bitplane_threads", C.c_uint32),
+                ("lane_pad", C.c_uint32)]
 
 
 class rf_stats(C.Structure):
@@ -72,6 +73,9 @@ def runtime() -> C.CDLL:
         "rf_submit_queue_bitplane": (C.c_int, [P, u64, P, u32, P, P]),
         "rf_set_mem_images": (C.c_int, [P, u32, u32, u32, P]),
         "rf_lane_state": (C.c_int, [P, u32, P, C.POINTER(C.c_size_t)]),
+        "rf_checkpoint_bytes": (C.c_int, [P, C.POINTER(C.c_size_t)]),
+        "rf_checkpoint_save": (C.c_int, [P, P, C.c_size_t]),
+        "rf_checkpoint_load": (C.c_int, [P, P, C.c_size_t]),
         "rf_profile_mtasks": (C.c_int, [P, u32, P]),
         "rf_sync": (C.c_int, [P]),
         "rf_flags": (C.c_int, [P, P, u32]),
@@ -128,11 +132,12 @@ class Model:
     """One design instantiated for ``n_lanes`` stimuli on the current GPU."""
 
     def __init__(self, design: DesignLib, n_lanes: int, settle_unroll: int = 2,
-                 converge_limit: int = 100, fused_chunk: int = 512, bitplane_threads: int = 0):
+                 converge_limit: int = 100, fused_chunk: int = 512, bitplane_threads: int = 0,
+                 lane_pad: int = 0):
         self.rt = runtime()
         self.design = design
         self.n = int(n_lanes)
-        cfg = rf_config(settle_unroll, converge_limit, fused_chunk, bitplane_threads)
+        cfg = rf_config(settle_unroll, converge_limit, fused_chunk, bitplane_threads, lane_pad)
         h = C.c_void_p()
         _check(self.rt.rf_create(design.desc, self.n, C.byref(cfg), C.byref(h)))
         self.h = h
@@ -245,6 +250,33 @@ class Model:
         buf = (C.c_uint8 * n.value)()
         _check(self.rt.rf_lane_state(self.h, lane, buf, C.byref(n)))
         return bytes(buf)
+
+    # ------------------------------------------------------- checkpoints
+    def checkpoint_bytes(self) -> int:
+        n = C.c_size_t(0)
+        _check(self.rt.rf_checkpoint_bytes(self.h, C.byref(n)))
+        return n.value
+
+    def checkpoint(self) -> bytes:
+        """The model's whole simulation state as a host blob."""
+        n = self.checkpoint_bytes()
+        buf = (C.c_uint8 * n)()
+        _check(self.rt.rf_checkpoint_save(self.h, buf, n))
+        return bytes(buf)
+
+    def restore(self, blob: bytes):
+        buf = C.create_string_buffer(blob, len(blob))
+        _check(self.rt.rf_checkpoint_load(self.h, buf, len(blob)))
+        self.sync()
+
+    def checkpoint_to(self, ptr: int, nbytes: int):
+        """Save into caller-owned memory at ``ptr`` (host or device)."""
+        _check(self.rt.rf_checkpoint_save(self.h, C.c_void_p(ptr), nbytes))
+
+    def restore_from(self, ptr: int, nbytes: int):
+        """Queue a restore from caller-owned memory at ``ptr`` (host or
+        device) on the model's stream."""
+        _check(self.rt.rf_checkpoint_load(self.h, C.c_void_p(ptr), nbytes))
 
     def profile_mtasks(self, reps: int = 20) -> np.ndarray:
         """Mean kernel time (ms) of every mtask of one eval, each timed alone."""

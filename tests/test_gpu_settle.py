@@ -113,3 +113,60 @@ def test_gated_clock(gpu, mode):
     with cd.model(8) as m:
         got = m.submit_queue(seed, budgets)
     assert (got == _gated_want(seed, len(budgets), 0, budgets)).all()
+
+
+def _smear_want(seed, n, cycles, budgets=None):
+    from rtlflow_amd.designs import settle
+    out = np.zeros(n, np.uint64)
+    for s in range(n):
+        h = 0xcbf29ce484222325
+        cyc = cycles if budgets is None else int(budgets[s])
+        for b in settle.smear_closed_form(oracle.stim, seed, s, cyc):
+            h = oracle.fnv(b, h)
+        out[s] = h
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [MODE_GRAPH, MODE_FUSED])
+def test_deep_settle_protocol(gpu, mode):
+    """smear needs up to 17 settle passes per eval: the graph form's two
+    captured passes are finished by its settle-tail kernel, with the default
+    configuration, and every form matches the closed form."""
+    cd = registry.get("smear")
+    seed, n, cycles = 0x5EA1, 200, 40
+    with cd.model(n) as m:
+        m.begin(seed)
+        m.run_cycles(cycles, mode)
+        got = m.hashes()
+    assert (got == _smear_want(seed, n, cycles)).all()
+
+
+@pytest.mark.gpu
+def test_deep_settle_eval_api_and_queue(gpu):
+    from rtlflow_amd.designs import settle
+    cd = registry.get("smear")
+    seed, n, cycles = 0x5EA1, 64, 8
+    r = np.zeros(n, np.uint32)
+    with cd.model(n) as m:
+        for c in range(cycles):
+            a = np.array([oracle.stim(seed, s, c, 0) & 15 for s in range(n)], np.uint8)
+            m.set_port("a", a)
+            m.set_port("clk", np.zeros(n, np.uint8))
+            m.eval()
+            m.set_port("clk", np.ones(n, np.uint8))
+            m.eval()
+            x = (0xFFFF << a.astype(np.uint32)) & 0xFFFF
+            r = (r + x) & 0xFFFF
+            assert (m.get_port("ox") == x).all() and (m.get_port("o") == r).all()
+    budgets = np.array([40, 0, 3, 17, 29] * 8, np.uint32)
+    with cd.model(16) as m:
+        got = m.submit_queue(seed, budgets)
+    assert (got == _smear_want(seed, len(budgets), 0, budgets)).all()
+    # a converge limit below the passes the design needs is reported by the
+    # graph form's tail as by the other forms
+    with cd.model(64, converge_limit=3) as m:
+        m.begin(seed)
+        m.run_cycles(4, MODE_GRAPH)
+        with pytest.raises(RuntimeError, match="converge"):
+            m.hashes()

@@ -147,6 +147,50 @@ def test_settle_loop_host_build():
         HostDesign(Compiled(elaborate(settle.OSC), Options(threads=2))).protocol(2, seed, 3, mode=1)
 
 
+def _smear_want(seed, nstim, cycles, budgets=None):
+    from rtlflow_amd.designs import settle
+    out = []
+    for s in range(nstim):
+        h = 0xcbf29ce484222325
+        cyc = cycles if budgets is None else int(budgets[s])
+        for b in settle.smear_closed_form(oracle.stim, seed, s, cyc):
+            h = oracle.fnv(b, h)
+        out.append(h)
+    return out
+
+
+def test_deep_settle_host_build():
+    """smear needs up to 17 settle passes per eval, more than the graph
+    form's two captured passes: the settle tail finishes them. Closed form,
+    interpreter, the host loop, the fused kernel, the graph-form schedule
+    (two passes + tail) and the batched queue agree."""
+    import ir_interp
+    from rtlflow_amd.designs import settle
+    seed, n, cycles = 0x5EA1, 6, 40
+    passes = [settle.smear_passes(oracle.stim, seed, s, cycles) for s in range(n)]
+    assert max(max(p) for p in passes) >= 8 and sum(p > 4 for q in passes for p in q) >= 10
+    want = _smear_want(seed, n, cycles)
+    assert ir_interp.protocol(elaborate(settle.SMEAR), seed, n, cycles) == want
+    comp = Compiled(elaborate(settle.SMEAR), Options(threads=2))
+    assert comp.sch.change_vars
+    hd = HostDesign(comp)
+    for mode in (0, 1, 2, 3):
+        assert list(hd.protocol(n, seed, cycles, mode=mode, order_seed=4)) == want, mode
+    budgets = [40, 0, 9, 33, 1, 17]
+    assert list(hd.queue(3, seed, budgets)) == _smear_want(seed, len(budgets), 0, budgets)
+
+
+def test_graph_schedule_tail_reports_unsettled():
+    """The graph-form schedule's settle tail reports a loop that never
+    settles, and leaves designs without combinational cycles unchanged."""
+    from rtlflow_amd.designs import settle
+    seed = 0x5E77
+    with pytest.raises(RuntimeError):
+        HostDesign(Compiled(elaborate(settle.OSC), Options(threads=2))).protocol(2, seed, 3, mode=3)
+    hd = HostDesign(Compiled(elaborate(settle.CIRC), Options(threads=2)))
+    assert list(hd.protocol(6, seed, 20, mode=3)) == _circ_closed_form(seed, 6, 20)
+
+
 def test_gated_clock_host_build():
     """A derived (gated) clock, clk & en: its logic runs before the block it
     clocks, whose edge test sees the new value; closed form, interpreter and
