@@ -116,7 +116,7 @@ def main():
     ap.add_argument("--learned-cycles", type=int, default=64,
                     help="cycles of the learned-vs-heuristic partition comparison (0: skip)")
     ap.add_argument("--workload", default="netlist",
-                    choices=["netlist", "alu", "systolic", "rv32i", "net5s"],
+                    choices=["netlist", "alu", "systolic", "rv32i", "net5s", "net5"],
                     help="netlist (default, the headline) or one of the other synthetic designs")
     ap.add_argument("--check", type=int, default=64, help="stimuli checked per sampled group")
     ap.add_argument("--cpu-sample", type=int, default=64,
@@ -339,116 +339,164 @@ def bench_netlist(args, torch):
     print(json.dumps(line), flush=True)
 
 
+# memory bytes a stimulus-cycle touches by design (T_mem): ALU two register
+# reads and one write of 4 B; RV32I a 4 B fetch, at most 4 B of data and the
+# register file's two reads and one write; systolic none
+T_MEM = {"alu": 12, "systolic": 0, "rv32i": 20}
+
+
+def roofline_line(b_alg, stim_cycles_per_launch, per_launch_ms, kernel, traffic_json=None, shape=None):
+    achieved = b_alg * stim_cycles_per_launch / (per_launch_ms * 1e-3) / 1e9
+    traffic = None
+    t = profile_json(traffic_json) if traffic_json else None
+    if t and shape and all(t.get(k) == v for k, v in shape.items()):
+        traffic = t["read_bytes"] + t["write_bytes"]
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
+            "alg_bytes_per_stimulus_cycle": b_alg, "avg_launch_ms": per_launch_ms}
+
+
 def bench_other(args):
     """The other synthetic designs at their own sizes: ALU 1,024 x 10,000,
-    systolic 8,192 x 5,000 (graph and fused forms), RV32I 16,384 stimuli with
-    lognormal budgets through the batched queue, beside the graph form at the
-    median budget."""
+    systolic 8,192 x 5,000 (fused, tuned and graph forms), RV32I 16,384
+    stimuli with lognormal budgets through the batched queue, beside the
+    graph form at the median budget. A step is the whole workload."""
     from rtlflow_amd.designs import registry, rv32i
     from rtlflow_amd.model import MODE_FUSED, MODE_GRAPH
     import oracle
     wl = args.workload
     if wl == "net5s":
         return bench_net5s(args)
+    if wl == "net5":
+        return bench_net5(args)
     cfg = {"alu": 1, "rv32i": 3, "systolic": 4}[wl]
     seed = registry.SEED_CFG[cfg]
     cd = registry.get(wl)
     man = cd.manifest
+    b_alg = 2 * man["bytes_seq"] + T_MEM[wl]
+    th = cpu_threads()
     forms = {}
     if wl in ("alu", "systolic"):
         lanes, cycles = {"alu": (1024, 10000), "systolic": (8192, 5000)}[wl]
-        cps = min(cycles, args.cycles_per_step * 10)
-        steps = cycles // cps
         run = oracle.alu_run if wl == "alu" else oracle.systolic_run
         nchk = min(args.check, lanes)
-        ref = run(seed, nchk, cycles, threads=16)
-        runs = [("fused", cd, MODE_FUSED, args.chunk), ("graph", cd, MODE_GRAPH, args.chunk)]
+        ref = np.concatenate([run(seed, nchk, cycles, threads=th),
+                              run(seed, nchk, cycles, stim0=lanes - nchk, threads=th)])
+        runs = [("fused", cd, MODE_FUSED, args.chunk, 0), ("graph", cd, MODE_GRAPH, args.chunk, 0)]
         if f"{wl}_tuned" in registry.load_index():
-            # the autotuner's compile-time choice (stimuli per block, memory layout)
+            # the autotuner's choice of compile-time options and launch shape
             from rtlflow_amd.autotune import choice
             t = choice(wl)
             runs.append(("tuned", registry.get(f"{wl}_tuned"), MODE_FUSED,
-                         t["knobs"].get("chunk", args.chunk)))
-        for form, c, mode, chunk in runs:
+                         t["knobs"].get("chunk", args.chunk), t["knobs"].get("lane_pad", 0)))
+        for form, c, mode, chunk, pad in runs:
             log(f"[bench] {wl} form={form} lanes={lanes} cycles={cycles}")
-            dt, st, h = run_form(c, lanes, mode, steps, cps, args.warmup, seed, chunk)
-            ok = bool((h[:nchk] == ref).all())
+            with c.model(lanes, fused_chunk=chunk, lane_pad=pad) as m:
+                m.begin(seed)
+                m.run_cycles(0, mode)
+                m.sync()
+                m.reset_stats()
+                t0 = time.perf_counter()
+                m.run_cycles(cycles, mode)
+                m.sync()
+                dt = time.perf_counter() - t0
+                st = m.stats()
+                h = m.hashes()
+            got = np.concatenate([h[:nchk], h[lanes - nchk:]])
+            ok = bool((got == ref).all())
+            if mode == MODE_FUSED:
+                per = st["fused_ms"] / max(1, st["fused_launches"])
+                rl = roofline_line(b_alg, lanes * min(chunk, cycles), per, "rf_k_fused",
+                                   f"{wl}_fused_traffic.json",
+                                   {"lanes": lanes, "cycles_per_launch": min(chunk, cycles)})
+            else:
+                per = st["graph_ms"] / max(1, st["graph_launches"])
+                rl = roofline_line(b_alg, lanes, per, "cycle graph")
             forms[form] = {"seconds": dt, "value": lanes * cycles / dt, "bit_exact_sample": ok,
-                           "checked_stimuli": nchk}
+                           "checked_stimuli": 2 * nchk, "roofline": rl}
             log(f"[bench] form={form} {dt:.3f}s {lanes * cycles / dt:.4g} stim-cycles/s oracle_match={ok}")
         workload = f"{wl}: {lanes} stimuli x {cycles} cycles"
+        ns = args.cpu_sample
         t0 = time.perf_counter()
-        run(seed, args.cpu_sample, cycles, threads=16)
+        run(seed, ns, cycles, threads=th)
         cdt = time.perf_counter() - t0
-        cpu = {"value": args.cpu_sample * cycles / cdt, "unit": "stimulus-cycles/s", "cores": 16,
-               "kind": "port", "sample": f"{args.cpu_sample} stimuli x {cycles} cycles, C oracle"}
+        cpu = {"value": ns * cycles / cdt, "unit": "stimulus-cycles/s", "cores": th,
+               "kind": "port", "nproc": os.cpu_count(),
+               "sample": f"{ns} stimuli x {cycles} cycles, C oracle ({cdt:.1f} s)"}
     else:
         nstim, lanes = 16384, args.lanes if args.lanes != 65536 else 8192
         progs = rv32i.programs(seed, np.arange(nstim))
         budgets = rv32i.budgets(seed, nstim)
         total = int(budgets.astype(np.int64).sum())
         nchk = min(args.check, nstim)
-        ref = oracle.rv32i_run(progs[:nchk], seed, budgets[:nchk], threads=16)
-        log(f"[bench] rv32i batched queue: {nstim} stimuli on {lanes} lanes, {total} stimulus-cycles")
-        with cd.model(lanes) as m:
-            m.set_mem_images("imem", progs)
-            m.sync()
-            t0 = time.perf_counter()
-            out = m.submit_queue(seed, budgets)
-            dt = time.perf_counter() - t0
-        ok = bool((out[:nchk] == ref).all())
-        forms["batched"] = {"seconds": dt, "value": total / dt, "bit_exact_sample": ok,
-                            "checked_stimuli": nchk, "lanes": lanes}
-        log(f"[bench] batched {dt:.3f}s {total / dt:.4g} stim-cycles/s oracle_match={ok}")
+        ref = oracle.rv32i_run(progs[:nchk], seed, budgets[:nchk], threads=th)
+        libs = [("batched", cd, lanes, 0)]
         if "rv32i_tuned" in registry.load_index():
-            # the autotuner's choice: stimuli per block, memory layout, lanes per
-            # queue (measured on a quarter-size queue, scaled to this one)
+            # the autotuner's choice: stimuli per block, memory layout, lanes
+            # per queue (measured on a quarter-size queue, scaled to this one)
             from rtlflow_amd.autotune import WORKLOADS, choice
             t = choice("rv32i")
-            tl = t["knobs"]["lanes"] * nstim // WORKLOADS["rv32i"]["nstim"]
-            with registry.get("rv32i_tuned").model(tl) as m:
+            libs.append(("tuned", registry.get("rv32i_tuned"),
+                         t["knobs"]["lanes"] * nstim // WORKLOADS["rv32i"]["nstim"],
+                         t["knobs"].get("lane_pad", 0)))
+        outs = {}
+        for form, c, ln, pad in libs:
+            log(f"[bench] rv32i {form} queue: {nstim} stimuli on {ln} lanes, {total} stimulus-cycles")
+            with c.model(ln, lane_pad=pad) as m:
                 m.set_mem_images("imem", progs)
                 m.sync()
+                m.reset_stats()
                 t0 = time.perf_counter()
-                out2 = m.submit_queue(seed, budgets)
-                dt2 = time.perf_counter() - t0
-            ok2 = bool((out2 == out).all())
-            forms["tuned"] = {"seconds": dt2, "value": total / dt2, "bit_exact_sample": ok2 and ok,
-                              "checked_stimuli": nchk, "lanes": tl}
-            log(f"[bench] tuned batched {dt2:.3f}s {total / dt2:.4g} stim-cycles/s same_hashes={ok2}")
+                out = m.submit_queue(seed, budgets)
+                dt = time.perf_counter() - t0
+                st = m.stats()
+            outs[form] = out
+            ok = bool((out[:nchk] == ref).all()) and bool((out == outs["batched"]).all())
+            rl = roofline_line(b_alg, total, st["fused_ms"] / max(1, st["fused_launches"]),
+                               "rf_k_persistent")
+            forms[form] = {"seconds": dt, "value": total / dt, "bit_exact_sample": ok,
+                           "checked_stimuli": nchk, "lanes": ln, "roofline": rl}
+            log(f"[bench] {form} {dt:.3f}s {total / dt:.4g} stim-cycles/s oracle_match={ok}")
         med = int(np.median(budgets))
         gl = 4096
         with cd.model(gl) as m:
             m.mem_write("imem", progs[:gl])
             m.begin(seed)
+            m.run_cycles(0, MODE_GRAPH)
             m.sync()
+            m.reset_stats()
             t0 = time.perf_counter()
             m.run_cycles(med, MODE_GRAPH)
             m.sync()
             dtg = time.perf_counter() - t0
+            st = m.stats()
             hg = m.hashes()
-        refg = oracle.rv32i_run(progs[:8], seed, [med] * 8, threads=8)
+        refg = oracle.rv32i_run(progs[:8], seed, [med] * 8, threads=th)
         forms["graph"] = {"seconds": dtg, "value": gl * med / dtg, "bit_exact_sample":
                           bool((hg[:8] == refg).all()), "checked_stimuli": 8, "lanes": gl,
-                          "cycles": med}
+                          "cycles": med,
+                          "roofline": roofline_line(b_alg, gl, st["graph_ms"] / max(1, st["graph_launches"]),
+                                                    "cycle graph")}
         log(f"[bench] graph {dtg:.3f}s {gl * med / dtg:.4g} stim-cycles/s")
         workload = f"rv32i: {nstim} stimuli, lognormal budgets (median 5000, [1000, 100000]), {total} cycles"
         ns = args.cpu_sample
         t0 = time.perf_counter()
-        oracle.rv32i_run(progs[:ns], seed, budgets[:ns], threads=16)
+        oracle.rv32i_run(progs[:ns], seed, budgets[:ns], threads=th)
         cdt = time.perf_counter() - t0
         cpu = {"value": int(budgets[:ns].astype(np.int64).sum()) / cdt, "unit": "stimulus-cycles/s",
-               "cores": 16, "kind": "port", "sample": f"{ns} stimuli at their budgets, C oracle"}
+               "cores": th, "kind": "port", "nproc": os.cpu_count(),
+               "sample": f"{ns} stimuli at their budgets, C oracle ({cdt:.1f} s)"}
     exact = {k: v for k, v in forms.items() if v["bit_exact_sample"]}
     main_form = max(exact or forms, key=lambda k: forms[k]["value"])
     f = forms[main_form]
     line = {
         "metric": "stimulus-cycles/sec, bit-exact vs Verilator CPU; batched+learned vs plain GPU",
         "value": f["value"], "unit": "stimulus-cycles/s", "n_gpus": 1, "steps": 1,
-        "warmup": args.warmup, "ms_per_step": f["seconds"] * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "integer", "data": "synthetic",
+        "warmup": 0, "ms_per_step": f["seconds"] * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": workload, "design": wl, "form": main_form, "mtasks": man["mtasks"]},
         "forms": forms, "bit_exact_sample": all(v["bit_exact_sample"] for v in forms.values()),
+        "roofline": f["roofline"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -470,7 +518,7 @@ def bench_net5s(args):
     total = int(budgets.astype(np.int64).sum())
     tb = registry.table_bytes("net5s")
     nchk = min(args.check, nstim)
-    ref = oracle.netlist_run(tb, seed, nchk, budgets=budgets[:nchk], threads=16)
+    ref = oracle.netlist_run(tb, seed, nchk, budgets=budgets[:nchk], threads=cpu_threads())
     forms = {}
     hashes = {}
     # the per-lane batched forms take about 7 ms per cycle at this size (one
@@ -478,7 +526,7 @@ def bench_net5s(args):
     # capped at the median, heuristic and learned partitions alike
     capb = np.minimum(budgets, 2000).astype(np.uint32)
     ctotal = int(capb.astype(np.int64).sum())
-    cref = oracle.netlist_run(tb, seed, nchk, budgets=capb[:nchk], threads=16)
+    cref = oracle.netlist_run(tb, seed, nchk, budgets=capb[:nchk], threads=cpu_threads())
     for form, name in (("batched", "net5s"), ("batched_learned", "net5s_learned")):
         cd = registry.get(name)
         log(f"[bench] net5s {form}: {nstim} stimuli on {lanes} lanes, budgets capped at 2000 "
@@ -509,13 +557,20 @@ def bench_net5s(args):
         with cd.model(lanes, bitplane_threads=nt) as m:
             m.submit_queue(seed, budgets[:64], mode=MODE_BITPLANE)    # code load
             m.sync()
+            m.reset_stats()
             t0 = time.perf_counter()
             out = m.submit_queue(seed, budgets, mode=MODE_BITPLANE)
             dt = time.perf_counter() - t0
+            st = m.stats()
         ok = bool((out[:nchk] == ref).all())
+        # one launch per round of lanes; the useful stimulus-cycles (each
+        # stimulus's own budget) over the launches
+        b_alg = 2 * ((cd.manifest["bitplane"]["state_bits"] + 7) // 8) + 8 * 16
+        rl = roofline_line(b_alg, total / max(1, st["bitplane_launches"]),
+                           st["bitplane_ms"] / max(1, st["bitplane_launches"]), f"rf_bp_kernel<{nt}>")
         forms["bitplane"] = {"seconds": dt, "value": total / dt, "bit_exact_sample": ok,
                              "checked_stimuli": nchk, "lanes": lanes, "threads": nt,
-                             "stimulus_cycles": total}
+                             "stimulus_cycles": total, "roofline": rl}
         log(f"[bench] bitplane {dt:.3f}s {total / dt:.4g} stim-cycles/s oracle_match={ok}")
     # the learned partition in the form its cost model predicts (the
     # per-mtask graph form: the sum of an eval's mtask kernel times)
@@ -559,10 +614,11 @@ def bench_net5s(args):
     log(f"[bench] graph {dtg:.3f}s {nstim * med / dtg:.4g} stim-cycles/s")
     ns = args.cpu_sample
     t0 = time.perf_counter()
-    oracle.netlist_run(tb, seed, ns, budgets=budgets[:ns], threads=16)
+    oracle.netlist_run(tb, seed, ns, budgets=budgets[:ns], threads=cpu_threads())
     cdt = time.perf_counter() - t0
     cpu = {"value": int(budgets[:ns].astype(np.int64).sum()) / cdt, "unit": "stimulus-cycles/s",
-           "cores": 16, "kind": "port", "sample": f"{ns} stimuli at their budgets, C oracle"}
+           "cores": cpu_threads(), "kind": "port", "nproc": os.cpu_count(),
+           "sample": f"{ns} stimuli at their budgets, C oracle"}
     exact = {k: v for k, v in forms.items() if v["bit_exact_sample"]}
     main_form = max(exact or forms, key=lambda k: forms[k]["value"])
     f = forms[main_form]
@@ -578,7 +634,97 @@ def bench_net5s(args):
         "forms": forms, "bit_exact_sample": all(v["bit_exact_sample"] for v in forms.values()),
         "learned_vs_heuristic": forms["batched_learned"]["value"] / forms["batched"]["value"],
         "learned_partition": learned,
+        "roofline": f.get("roofline"),
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_net5(args):
+    """Configuration 5 at its stated size through the gate-net engine: a
+    random netlist of 500,000 gates, 50,000 flip-flops, eight 1,024 x 64-bit
+    synchronous RAMs, 128 inputs and 64 outputs; 4,096 stimuli with
+    lognormal budgets (median 2,000, clipped to [200, 20,000]). The
+    bit-plane form runs the whole workload (a step); the plain per-mtask
+    graph form runs the heuristic and the learned partitions on a bounded
+    number of cycles, and the heuristic judges the learned one."""
+    from rtlflow_amd import judge
+    from rtlflow_amd.designs import registry, rv32i
+    from rtlflow_amd.gatenet import GateNet
+    import oracle
+    seed = registry.SEED_CFG[5]
+    nstim = 4096
+    budgets = rv32i.budgets(seed, nstim, median=2000, lo=200, hi=20000)
+    total = int(budgets.astype(np.int64).sum())
+    tb = registry.gatenet_table("net5")
+    meta = registry.gatenet_meta("net5")
+    th = cpu_threads()
+    forms = {}
+    log(f"[bench] net5 bitplane: {nstim} stimuli, {total} stimulus-cycles, max budget {int(budgets.max())}")
+    with GateNet(registry.gatenet_program("net5"), nstim) as g:
+        g.run(GateNet.BITPLANE, seed, np.minimum(budgets, 2))     # graph build, code load
+        t0 = time.perf_counter()
+        h, ms = g.run(GateNet.BITPLANE, seed, budgets)
+        dt = time.perf_counter() - t0
+        nodes = g.nodes_per_cycle(GateNet.BITPLANE)
+    nchk = min(args.check, 16)
+    ref = np.concatenate([oracle.netlist_run(tb, seed, nchk, budgets=budgets[:nchk], threads=th),
+                          oracle.netlist_run(tb, seed, nchk, stim0=nstim - nchk,
+                                             budgets=budgets[nstim - nchk:], threads=th)])
+    ok = bool((np.concatenate([h[:nchk], h[nstim - nchk:]]) == ref).all())
+    # algorithmic bytes: every flip-flop bit read and written once per
+    # stimulus-cycle, the RAM words read and written (8 x 2 x 8 B), the
+    # inputs and outputs (16 B + 8 B)
+    b_alg = 2 * ((50000 + 7) // 8) + 8 * 16 + 24
+    rl = roofline_line(b_alg, total, ms, "gate-net bit-plane cycle graph")
+    forms["bitplane"] = {"seconds": dt, "value": total / dt, "bit_exact_sample": ok, "checked_stimuli": 2 * nchk,
+                         "kernel_ms": ms, "graph_nodes_per_cycle": nodes, "roofline": rl}
+    log(f"[bench] net5 bitplane {dt:.3f}s {total / dt:.4g} stim-cycles/s oracle_match={ok}")
+    # the plain per-mtask graph form: heuristic and learned partitions on
+    # the first cycles of the same stimuli
+    gc = 8
+    gb = np.minimum(budgets, gc)
+    gref = oracle.netlist_run(tb, seed, nchk, budgets=gb[:nchk], threads=th)
+    res = {}
+    for tag in ("heuristic", "learned"):
+        if tag not in meta["schedules"]:
+            continue
+        with GateNet(registry.gatenet_program("net5", tag), nstim) as g:
+            g.run(GateNet.GRAPH, seed, np.minimum(budgets, 1))
+            t0 = time.perf_counter()
+            hg, gms = g.run(GateNet.GRAPH, seed, gb)
+            gdt = time.perf_counter() - t0
+            mt = g.mtasks()
+        res[tag] = (int(gb.astype(np.int64).sum()) / gdt, hg, mt)
+        forms[f"graph_{tag}"] = {"seconds": gdt, "value": res[tag][0],
+                                 "bit_exact_sample": bool((hg[:nchk] == gref).all()), "checked_stimuli": nchk,
+                                 "cycles": gc, "mtasks": mt, "threads": meta["schedules"][tag]["threads"]}
+        log(f"[bench] net5 graph {tag} ({mt} mtasks) {gdt:.3f}s {res[tag][0]:.4g} stim-cycles/s")
+    learned = None
+    if "learned" in res:
+        learned = judge.decide(res["heuristic"][0], res["learned"][0], res["heuristic"][1], res["learned"][1],
+                               gref, np.arange(nchk))
+        learned.update(form="graph", cycles=gc, heuristic_mtasks=res["heuristic"][2],
+                       learned_mtasks=res["learned"][2])
+    ns = 4
+    t0 = time.perf_counter()
+    oracle.netlist_run(tb, seed, ns, budgets=np.minimum(budgets[:ns], 400), threads=th)
+    cdt = time.perf_counter() - t0
+    csc = int(np.minimum(budgets[:ns], 400).astype(np.int64).sum())
+    cpu = {"value": csc / cdt, "unit": "stimulus-cycles/s", "cores": min(th, ns), "kind": "port",
+           "nproc": os.cpu_count(),
+           "sample": f"{ns} stimuli, budgets capped at 400 ({csc} stimulus-cycles, {cdt:.1f} s), C oracle"}
+    f = forms["bitplane"]
+    line = {
+        "metric": "stimulus-cycles/sec, bit-exact vs Verilator CPU; batched+learned vs plain GPU",
+        "value": f["value"], "unit": "stimulus-cycles/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": f["seconds"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"net5: random netlist 500k gates / 50k flops / 8 x 1,024 x 64-bit RAMs / "
+                               f"128 PI / 64 PO, {nstim} stimuli, lognormal budgets (median 2000, "
+                               f"[200, 20000]), {total} cycles", "design": "net5", "form": "bitplane"},
+        "forms": forms, "bit_exact_sample": all(v["bit_exact_sample"] for v in forms.values()),
+        "learned_partition": learned, "roofline": f["roofline"], "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

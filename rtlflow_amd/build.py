@@ -57,9 +57,10 @@ def runtime_path() -> Path:
 
 def build_runtime(force: bool = False) -> Path:
     out = runtime_path()
-    srcs = [CSRC / "rf_runtime.hip", CSRC / "rf_mlp.hip", CSRC / "rf_bitplane.hip"]
+    srcs = [CSRC / "rf_runtime.hip", CSRC / "rf_mlp.hip", CSRC / "rf_bitplane.hip", CSRC / "rf_gatenet.hip"]
     deps = srcs + [INCLUDE / "rf_verilated.h", INCLUDE / "rf_design.h", INCLUDE / "rf_mlp.h",
-                   CSRC / "rf_bitplane.h", CSRC / "rf_stim.h"]
+                   INCLUDE / "rf_gatenet.h", CSRC / "rf_bitplane.h", CSRC / "rf_stim.h",
+                   CSRC / "rf_transpose.h"]
     if out.exists() and not force and all(out.stat().st_mtime >= p.stat().st_mtime for p in deps):
         return out
     BUILD.mkdir(parents=True, exist_ok=True)

@@ -51,6 +51,17 @@ const size_t kClassBytes[4] = {1, 2, 4, 8};
 
 }  // namespace
 
+// The error channel of every entry point in the library (rf_last_error).
+int rf_set_error(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
 struct rf_model {
     const rf_design* d = nullptr;
     rf_config cfg{};

@@ -276,3 +276,86 @@ def _kat(family: str):
 
 for _f in ("shift", "div", "signed", "red"):
     register(f"kat_{_f}", _kat(_f), threads=2)
+
+
+# ---------------------------------------------------------------------------
+# gate-level netlists run by the gate-net engine (include/rf_gatenet.h):
+# configuration 5 at its stated size, and a small one of the same shape for
+# the GPU parity tests. Each build writes the gate table and the program
+# blobs of the heuristic partition (--threads 4) and, when the cost model
+# has chosen one (costtables/partition_choice.json), the learned partition.
+GATENETS = {
+    "net5": dict(n_gates=500000, n_ff=50000, n_pi=128, n_po=64, depth=40, seed=SEED_CFG[5], n_rams=8,
+                 ram_depth=1024, ram_width=64),
+    "net5_small": dict(n_gates=6000, n_ff=600, n_pi=128, n_po=64, depth=40, seed=SEED_CFG[5] ^ 0x5A,
+                       n_rams=8, ram_depth=1024, ram_width=64),
+}
+GATENET_THREADS = 4
+
+
+def gatenet_dir(name: str) -> Path:
+    return BUILD / "gatenet" / name
+
+
+def build_gatenet(name: str, log=print) -> dict:
+    from .. import gatenet as gn
+    from . import netlist as nl
+    import numpy as np
+    t0 = time.time()
+    kw = GATENETS[name]
+    d = gatenet_dir(name)
+    d.mkdir(parents=True, exist_ok=True)
+    table = nl.generate(**kw)[1] if kw["n_gates"] <= 20000 else _table_only(kw)
+    tb = table.to_bytes()
+    (d / "gates.bin").write_bytes(tb)
+    t = gn.parse_table(tb)
+    rec = {"name": name, "gates": kw["n_gates"], "schedules": {}}
+    choice = json.loads((COSTTABLES / "partition_choice.json").read_text()).get(name)
+    scheds = {"heuristic": GATENET_THREADS}
+    if choice:
+        scheds["learned"] = int(choice["threads"])
+    for tag, th in scheds.items():
+        prog = gn.program(t, threads=th)
+        (d / f"{tag}.bin").write_bytes(prog.astype("<u4").tobytes())
+        rec["schedules"][tag] = {"threads": th, "mtasks": int(prog[15]), "words": int(len(prog))}
+    rec["build_s"] = round(time.time() - t0, 1)
+    (d / "meta.json").write_text(json.dumps(rec, indent=1))
+    if log:
+        log(f"built gate net {name}: {rec['schedules']} ({rec['build_s']} s)")
+    return rec
+
+
+def _table_only(kw):
+    """The generator's gate table without its Verilog text (half a million
+    assign statements the gate-net engine does not read)."""
+    from . import netlist as nl
+    saved = nl.to_verilog
+    nl.to_verilog = lambda t, name="": ""
+    try:
+        return nl.generate(**kw)[1]
+    finally:
+        nl.to_verilog = saved
+
+
+def build_gatenets(names=None, log=print) -> dict:
+    return {n: build_gatenet(n, log) for n in (names or GATENETS)}
+
+
+def gatenet_meta(name: str) -> dict:
+    p = gatenet_dir(name) / "meta.json"
+    if not p.exists():
+        raise RuntimeError(f"gate net {name!r} is not built; run __graft_entry__.build()")
+    return json.loads(p.read_text())
+
+
+def gatenet_table(name: str) -> bytes:
+    gatenet_meta(name)
+    return (gatenet_dir(name) / "gates.bin").read_bytes()
+
+
+def gatenet_program(name: str, schedule: str = "heuristic"):
+    import numpy as np
+    meta = gatenet_meta(name)
+    if schedule not in meta["schedules"]:
+        raise RuntimeError(f"gate net {name!r} has no {schedule} schedule")
+    return np.frombuffer((gatenet_dir(name) / f"{schedule}.bin").read_bytes(), "<u4").copy()

@@ -42,3 +42,24 @@ def test_error_path_without_design():
     rc = lib.rf_create(None, 16, None, ctypes.byref(out))
     assert rc == -1
     assert b"null design" in lib.rf_last_error()
+
+
+def test_runtime_exports_gatenet_symbols():
+    lib = ctypes.CDLL(str(build_runtime()))
+    names = declared("rf_gatenet.h")
+    assert "rf_gn_create" in names and "rf_gn_run" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_gatenet_rejects_a_bad_program_without_a_gpu():
+    import numpy as np
+    lib = ctypes.CDLL(str(build_runtime()))
+    lib.rf_gn_create.restype = ctypes.c_int
+    lib.rf_gn_create.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32,
+                                 ctypes.POINTER(ctypes.c_void_p)]
+    lib.rf_last_error.restype = ctypes.c_char_p
+    out = ctypes.c_void_p()
+    bad = np.zeros(64, np.uint32)
+    rc = lib.rf_gn_create(bad.ctypes.data, 64, 16, ctypes.byref(out))
+    assert rc == -1 and b"gate-net program" in lib.rf_last_error()

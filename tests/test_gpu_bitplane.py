@@ -160,3 +160,25 @@ def test_bitplane_refused_for_asynchronous_memory_reads(gpu):
         m.begin(1)
         with pytest.raises(Exception, match="no bit-plane program"):
             m.run_cycles(1, MODE_BITPLANE)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lanes,threads", [(49153, 512), (65536, 512)])
+def test_bitplane_more_groups_than_workgroup_slots(gpu, lanes, threads):
+    """net2 at more 64-lane groups than the chip holds at once (three
+    workgroups of 512 threads per CU, 768 slots): groups of the second round
+    and the ragged last group match the oracle, on a strided sample."""
+    cd = registry.get("net2")
+    cycles, seed = 100, registry.SEED_CFG[2]
+    ng = (lanes + 63) // 64
+    print(f"bitplane net2 lanes={lanes} groups={ng} cycles={cycles}", flush=True)
+    with cd.model(lanes, bitplane_threads=threads, fused_chunk=cycles) as m:
+        m.begin(seed)
+        m.run_cycles(cycles, MODE_BITPLANE)
+        got = m.hashes()
+    tb = registry.table_bytes("net2")
+    for g in sorted(x for x in {0, 383, 767, 768, 769, ng // 2, ng - 2, ng - 1} if x < ng):
+        s0 = g * 64
+        k = min(64, lanes - s0)
+        want = oracle.netlist_run(tb, seed, k, cycles, stim0=s0)
+        assert (got[s0:s0 + k] == want).all(), f"group {g}: {int((got[s0:s0 + k] != want).sum())} differ"

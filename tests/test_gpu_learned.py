@@ -1,11 +1,15 @@
-"""The learned partition (granularity chosen by the MLP) changes the mtask
-graph, not the results: graph-form hashes of net2 equal the heuristic
-partition's and the oracle's."""
+"""The learned partition (granularity chosen by the MLP) on the GPU: it
+changes the mtask graph, not the results, and the heuristic judges it --
+on net5s it is kept in the per-mtask graph form, where its predicted eval
+time is the sum of its mtask kernel times, and the heuristic stays in the
+batched form, where every mtask of a lane runs in one thread and more mtasks
+only add calls."""
 import numpy as np
 import pytest
 
 import oracle
-from rtlflow_amd.designs import registry
+from rtlflow_amd import judge
+from rtlflow_amd.designs import registry, rv32i
 from rtlflow_amd.model import MODE_GRAPH
 
 
@@ -23,3 +27,32 @@ def test_learned_partition_is_bit_exact(gpu):
     assert (out[0] == out[1]).all()
     ref = oracle.netlist_run(registry.table_bytes("net2"), seed, 32, cycles)
     assert (out[1][:32] == ref).all()
+
+
+@pytest.mark.gpu
+def test_judge_keeps_learned_graph_form_on_net5s(gpu):
+    seed, lanes, cycles = registry.SEED_CFG[5], 4096, 100
+    tb = registry.table_bytes("net5s")
+    sample = np.arange(16)
+    v = judge.judge(registry.get("net5s"), registry.get("net5s_learned"),
+                    lambda cd: judge.measure_graph(cd, lanes, cycles, seed),
+                    lambda s: oracle.netlist_run(tb, seed, len(s), cycles=cycles), sample)
+    print("net5s graph form:", v, flush=True)
+    assert v["bit_exact"]
+    assert v["kept"] and v["speedup"] > 1.05
+
+
+@pytest.mark.gpu
+def test_judge_falls_back_in_batched_form_on_net5s(gpu):
+    seed, nstim = registry.SEED_CFG[5], 2048
+    budgets = rv32i.budgets(seed, nstim, median=150, lo=20, hi=600)
+    tb = registry.table_bytes("net5s")
+    sample = np.arange(16)
+    v = judge.judge(registry.get("net5s"), registry.get("net5s_learned"),
+                    lambda cd: judge.measure_batched(cd, 1024, budgets, seed),
+                    lambda s: oracle.netlist_run(tb, seed, len(s), budgets=budgets[:len(s)]), sample)
+    print("net5s batched form:", v, flush=True)
+    assert v["bit_exact"]
+    assert not v["kept"] and v["choice"] == "heuristic"
+    assert judge.scheduled("net5s", "batched").manifest["mtasks"] == v["heuristic_mtasks"] or \
+        not judge.verdicts().get("net5s", {}).get("batched", {}).get("kept", False)
