@@ -115,6 +115,21 @@ int rf_hashes(rf_model* m, uint32_t lane0, uint32_t count, uint64_t* out);
 int rf_submit_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
                     const uint32_t* order, uint64_t* out_hashes);
 
+/* Largest batched workload rf_submit_queue_static takes. */
+#define RF_QUEUE_STATIC_MAX 65536u
+
+/* rf_submit_queue without allocating: no hipMalloc, hipFree or other
+ * allocating call. No counterpart in the reference; it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid it). The queue buffers (budgets, order, per-stimulus
+ * hashes) live in a fixed device array of RF_QUEUE_STATIC_MAX stimuli
+ * (16 B each, 1 MiB per GPU, held until the process ends); nstim above that
+ * is refused with RF_E_ARG. Same arguments, results and errors as
+ * rf_submit_queue. Every model on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (two host threads, two models) race. */
+int rf_submit_queue_static(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
+                           const uint32_t* order, uint64_t* out_hashes);
+
 /* The batched workload in the bit-plane form (RF_E_STATE if the design has
  * no bit-plane program or memory images are set): the stimuli are dealt to
  * the lanes in queue order (order may be NULL: longest budget first), in

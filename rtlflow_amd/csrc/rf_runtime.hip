@@ -19,6 +19,7 @@
 #include <deque>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -69,6 +70,9 @@ struct rf_model {
     uint64_t npad = 0;
     uint32_t grid = 0;
     hipStream_t stream = nullptr;
+    int device = 0;
+    uint8_t* queue_scratch = nullptr;  // this device's rf_queue_scratch (rf_submit_queue_static)
+    int queue_scratch_err = 0;         // hipGetSymbolAddress's error, if it failed
     void* cls[4] = {nullptr, nullptr, nullptr, nullptr};
     uint8_t* mem = nullptr;
     uint32_t* change = nullptr;
@@ -310,6 +314,45 @@ int timed_end(rf_model* m, hipEvent_t e0, uint32_t kind, uint32_t count) {
 
 int check_flags(rf_model* m);
 
+// Queue buffers of rf_submit_queue_static: a file-scope device array, one
+// copy per GPU for the life of the process. Layout: out_hash[max] (u64),
+// budget[max] (u32), order[max] (u32). In host code ``rf_queue_scratch``
+// names a host-side stand-in, never a device address: only the address that
+// hipGetSymbolAddress returns for the current device may reach a kernel or
+// a copy.
+static __device__ __align__(256) uint8_t rf_queue_scratch[(size_t)RF_QUEUE_STATIC_MAX * 16];
+
+constexpr size_t kScratchHashOff = 0;
+constexpr size_t kScratchBudgetOff = (size_t)RF_QUEUE_STATIC_MAX * 8;
+constexpr size_t kScratchOrderOff = (size_t)RF_QUEUE_STATIC_MAX * 12;
+constexpr int kMaxDevices = 64;
+
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// This device's address of rf_queue_scratch, looked up once per device with
+// that device current (called by rf_create on the model's device).
+hipError_t queue_scratch_for(int dev, uint8_t** out) {
+    *out = nullptr;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_done[dev]) {
+        void* p = nullptr;
+        g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(rf_queue_scratch));
+        g_scratch_ptr[dev] = g_scratch_err[dev] == hipSuccess ? (uint8_t*)p : nullptr;
+        g_scratch_done[dev] = true;
+    }
+    *out = g_scratch_ptr[dev];
+    return g_scratch_err[dev];
+}
+
+// Host staging of the static queue's order (no heap): guarded by its mutex.
+std::mutex g_static_order_mu;
+uint32_t g_static_order[RF_QUEUE_STATIC_MAX];
+uint64_t g_static_seen[RF_QUEUE_STATIC_MAX / 64];
+
 // rf_begin's per-lane protocol state, set on the device so that a begin
 // queues behind earlier launches without a host wait.
 __global__ void rf_rt_begin(uint64_t* hash, uint32_t* stim, uint32_t* cyc, uint64_t npad,
@@ -430,7 +473,14 @@ int rf_create(const rf_design* d, uint32_t n_lanes, const rf_config* cfg, rf_mod
         rf_destroy(m);
         return rc;
     };
-    hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    hipError_t e = hipGetDevice(&m->device);
+    if (e != hipSuccess) return cleanup(fail(RF_E_HIP, "hipGetDevice: %s", hipGetErrorString(e)));
+    m->queue_scratch_err = (int)queue_scratch_for(m->device, &m->queue_scratch);
+    // the bookkeeping of queued launches never grows past these, so waiting
+    // for them (rf_submit_queue_static included) allocates nothing
+    m->pend.reserve(kMaxPending + 1);
+    m->evfree.reserve(2 * kMaxPending + 2);
+    e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return cleanup(fail(RF_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e)));
     for (int c = 0; c < 4; ++c) {
         const size_t bytes = std::max<uint64_t>(1, d->slots[c]) * m->npad * kClassBytes[c];
@@ -860,6 +910,45 @@ int rf_submit_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_
     int qrc = queue_setup(m, budgets, nstim, order, ord);
     if (qrc) return qrc;
     return run_queue(m, seed, budgets, nstim, ord.data(), m->budget, m->order, m->out_hash, out_hashes);
+}
+
+int rf_submit_queue_static(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
+                           const uint32_t* order, uint64_t* out_hashes) {
+    if (!m || !budgets || !out_hashes) return fail(RF_E_ARG, "null argument");
+    if (nstim > RF_QUEUE_STATIC_MAX)
+        return fail(RF_E_ARG, "rf_submit_queue_static: %u stimuli exceed its fixed queue of %u "
+                    "(use rf_submit_queue)", nstim, (uint32_t)RF_QUEUE_STATIC_MAX);
+    if (!m->queue_scratch)
+        return fail(RF_E_HIP, "rf_submit_queue_static: no device address for the queue buffers "
+                    "(hipGetSymbolAddress: %s)", hipGetErrorString((hipError_t)m->queue_scratch_err));
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
+    if (!nstim) return RF_OK;
+    std::lock_guard<std::mutex> lk(g_static_order_mu);
+    uint32_t* ord = g_static_order;
+    if (order) {
+        memset(g_static_seen, 0, ((size_t)nstim + 63) / 64 * 8);
+        for (uint32_t k = 0; k < nstim; ++k) {
+            const uint32_t x = order[k];
+            if (x >= nstim || (g_static_seen[x >> 6] >> (x & 63)) & 1)
+                return fail(RF_E_ARG, "order is not a permutation of [0, %u)", nstim);
+            g_static_seen[x >> 6] |= 1ull << (x & 63);
+            ord[k] = x;
+        }
+    } else {
+        // longest budget first; ties by stimulus id (the order the
+        // allocating form's stable sort gives), sorted in place
+        for (uint32_t k = 0; k < nstim; ++k) ord[k] = k;
+        std::sort(ord, ord + nstim, [&](uint32_t x, uint32_t y) {
+            return budgets[x] != budgets[y] ? budgets[x] > budgets[y] : x < y;
+        });
+    }
+    uint8_t* base = m->queue_scratch;
+    return run_queue(m, seed, budgets, nstim, ord, (uint32_t*)(base + kScratchBudgetOff),
+                     (uint32_t*)(base + kScratchOrderOff), (uint64_t*)(base + kScratchHashOff),
+                     out_hashes);
 }
 
 int rf_submit_queue_bitplane(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,

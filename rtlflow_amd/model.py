@@ -71,6 +71,7 @@ def runtime() -> C.CDLL:
         "rf_hashes": (C.c_int, [P, u32, u32, P]),
         "rf_submit_queue": (C.c_int, [P, u64, P, u32, P, P]),
         "rf_submit_queue_bitplane": (C.c_int, [P, u64, P, u32, P, P]),
+        "rf_submit_queue_static": (C.c_int, [P, u64, P, u32, P, P]),
         "rf_set_mem_images": (C.c_int, [P, u32, u32, u32, P]),
         "rf_lane_state": (C.c_int, [P, u32, P, C.POINTER(C.c_size_t)]),
         "rf_checkpoint_bytes": (C.c_int, [P, C.POINTER(C.c_size_t)]),
