@@ -119,11 +119,14 @@ def main():
                     choices=["netlist", "alu", "systolic", "rv32i", "net5s", "net5"],
                     help="netlist (default, the headline) or one of the other synthetic designs")
     ap.add_argument("--check", type=int, default=64, help="stimuli checked per sampled group")
-    ap.add_argument("--cpu-sample", type=int, default=64,
-                    help="stimuli in the CPU baseline sample of the other workloads")
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="stimuli in the CPU baseline sample of the other workloads (0: the "
+                         "workload's default, sized to a few seconds of CPU work)")
     ap.add_argument("--cpu-sample-netlist", type=int, default=1024,
                     help="stimuli in the netlist CPU baseline sample (x the full cycles)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--main-only", action="store_true",
+                    help="other workloads: run only the reported form (for counter profiles)")
     ap.add_argument("--budget-s", type=float, default=300.0,
                     help="optional legs (other forms, learned partition, CPU baseline) are "
                          "skipped once this many seconds have passed")
@@ -389,6 +392,8 @@ def bench_other(args):
             t = choice(wl)
             runs.append(("tuned", registry.get(f"{wl}_tuned"), MODE_FUSED,
                          t["knobs"].get("chunk", args.chunk), t["knobs"].get("lane_pad", 0)))
+        if args.main_only:
+            runs = runs[-1:]
         for form, c, mode, chunk, pad in runs:
             log(f"[bench] {wl} form={form} lanes={lanes} cycles={cycles}")
             with c.model(lanes, fused_chunk=chunk, lane_pad=pad) as m:
@@ -416,7 +421,9 @@ def bench_other(args):
                            "checked_stimuli": 2 * nchk, "roofline": rl}
             log(f"[bench] form={form} {dt:.3f}s {lanes * cycles / dt:.4g} stim-cycles/s oracle_match={ok}")
         workload = f"{wl}: {lanes} stimuli x {cycles} cycles"
-        ns = args.cpu_sample
+        if args.main_only:
+            return print_other(args, wl, man, forms, workload, None)
+        ns = args.cpu_sample or lanes       # the whole workload: a second or two on the CPU
         t0 = time.perf_counter()
         run(seed, ns, cycles, threads=th)
         cdt = time.perf_counter() - t0
@@ -440,6 +447,8 @@ def bench_other(args):
                          t["knobs"]["lanes"] * nstim // WORKLOADS["rv32i"]["nstim"],
                          t["knobs"].get("lane_pad", 0)))
         outs = {}
+        if args.main_only:
+            libs = libs[-1:]
         for form, c, ln, pad in libs:
             log(f"[bench] rv32i {form} queue: {nstim} stimuli on {ln} lanes, {total} stimulus-cycles")
             with c.model(ln, lane_pad=pad) as m:
@@ -451,7 +460,7 @@ def bench_other(args):
                 dt = time.perf_counter() - t0
                 st = m.stats()
             outs[form] = out
-            ok = bool((out[:nchk] == ref).all()) and bool((out == outs["batched"]).all())
+            ok = bool((out[:nchk] == ref).all()) and bool((out == outs.get("batched", out)).all())
             rl = roofline_line(b_alg, total, st["fused_ms"] / max(1, st["fused_launches"]),
                                "rf_k_persistent")
             forms[form] = {"seconds": dt, "value": total / dt, "bit_exact_sample": ok,
@@ -459,6 +468,8 @@ def bench_other(args):
             log(f"[bench] {form} {dt:.3f}s {total / dt:.4g} stim-cycles/s oracle_match={ok}")
         med = int(np.median(budgets))
         gl = 4096
+        if args.main_only:
+            return print_other(args, wl, man, forms, f"rv32i: {nstim} stimuli, lognormal budgets", None)
         with cd.model(gl) as m:
             m.mem_write("imem", progs[:gl])
             m.begin(seed)
@@ -479,13 +490,17 @@ def bench_other(args):
                                                     "cycle graph")}
         log(f"[bench] graph {dtg:.3f}s {gl * med / dtg:.4g} stim-cycles/s")
         workload = f"rv32i: {nstim} stimuli, lognormal budgets (median 5000, [1000, 100000]), {total} cycles"
-        ns = args.cpu_sample
+        ns = args.cpu_sample or 4096
         t0 = time.perf_counter()
         oracle.rv32i_run(progs[:ns], seed, budgets[:ns], threads=th)
         cdt = time.perf_counter() - t0
         cpu = {"value": int(budgets[:ns].astype(np.int64).sum()) / cdt, "unit": "stimulus-cycles/s",
                "cores": th, "kind": "port", "nproc": os.cpu_count(),
                "sample": f"{ns} stimuli at their budgets, C oracle ({cdt:.1f} s)"}
+    return print_other(args, wl, man, forms, workload, cpu)
+
+
+def print_other(args, wl, man, forms, workload, cpu):
     exact = {k: v for k, v in forms.items() if v["bit_exact_sample"]}
     main_form = max(exact or forms, key=lambda k: forms[k]["value"])
     f = forms[main_form]
@@ -527,7 +542,7 @@ def bench_net5s(args):
     capb = np.minimum(budgets, 2000).astype(np.uint32)
     ctotal = int(capb.astype(np.int64).sum())
     cref = oracle.netlist_run(tb, seed, nchk, budgets=capb[:nchk], threads=cpu_threads())
-    for form, name in (("batched", "net5s"), ("batched_learned", "net5s_learned")):
+    for form, name in (() if args.main_only else (("batched", "net5s"), ("batched_learned", "net5s_learned"))):
         cd = registry.get(name)
         log(f"[bench] net5s {form}: {nstim} stimuli on {lanes} lanes, budgets capped at 2000 "
             f"({ctotal} stimulus-cycles), {cd.manifest['mtasks']} mtasks")
@@ -542,8 +557,9 @@ def bench_net5s(args):
                        "checked_stimuli": nchk, "lanes": lanes, "mtasks": cd.manifest["mtasks"],
                        "budgets": "capped at 2000", "stimulus_cycles": ctotal}
         log(f"[bench] {form} {dt:.3f}s {ctotal / dt:.4g} stim-cycles/s oracle_match={ok}")
-    same = bool((hashes["batched"] == hashes["batched_learned"]).all())
-    forms["batched_learned"]["same_hashes_as_heuristic"] = same
+    if not args.main_only:
+        same = bool((hashes["batched"] == hashes["batched_learned"]).all())
+        forms["batched_learned"]["same_hashes_as_heuristic"] = same
     cd = registry.get("net5s")
     if cd.manifest.get("bitplane", {}).get("supported"):
         # the bit-plane form: stimuli dealt to 64-lane groups longest budget
@@ -572,6 +588,12 @@ def bench_net5s(args):
                              "checked_stimuli": nchk, "lanes": lanes, "threads": nt,
                              "stimulus_cycles": total, "roofline": rl}
         log(f"[bench] bitplane {dt:.3f}s {total / dt:.4g} stim-cycles/s oracle_match={ok}")
+    if args.main_only:
+        f = forms["bitplane"]
+        print(json.dumps({"metric": "stimulus-cycles/sec, bit-exact vs Verilator CPU; batched+learned vs plain GPU",
+                          "value": f["value"], "unit": "stimulus-cycles/s", "n_gpus": 1, "forms": forms,
+                          "roofline": f.get("roofline")}), flush=True)
+        return
     # the learned partition in the form its cost model predicts (the
     # per-mtask graph form: the sum of an eval's mtask kernel times)
     lc = 200
@@ -612,7 +634,7 @@ def bench_net5s(args):
                       "bit_exact_sample": bool((hg[:8] == refg).all()), "checked_stimuli": 8,
                       "lanes": nstim, "cycles": med}
     log(f"[bench] graph {dtg:.3f}s {nstim * med / dtg:.4g} stim-cycles/s")
-    ns = args.cpu_sample
+    ns = args.cpu_sample or 128
     t0 = time.perf_counter()
     oracle.netlist_run(tb, seed, ns, budgets=budgets[:ns], threads=cpu_threads())
     cdt = time.perf_counter() - t0
@@ -633,6 +655,7 @@ def bench_net5s(args):
                    "design": "net5s", "form": main_form},
         "forms": forms, "bit_exact_sample": all(v["bit_exact_sample"] for v in forms.values()),
         "learned_vs_heuristic": forms["batched_learned"]["value"] / forms["batched"]["value"],
+        "judge": {"graph": learned, "batched": {"kept": forms["batched_learned"]["value"] > forms["batched"]["value"]}},
         "learned_partition": learned,
         "roofline": f.get("roofline"),
         "cpu_baseline": cpu,
@@ -680,6 +703,11 @@ def bench_net5(args):
     forms["bitplane"] = {"seconds": dt, "value": total / dt, "bit_exact_sample": ok, "checked_stimuli": 2 * nchk,
                          "kernel_ms": ms, "graph_nodes_per_cycle": nodes, "roofline": rl}
     log(f"[bench] net5 bitplane {dt:.3f}s {total / dt:.4g} stim-cycles/s oracle_match={ok}")
+    if args.main_only:
+        print(json.dumps({"metric": "stimulus-cycles/sec, bit-exact vs Verilator CPU; batched+learned vs plain GPU",
+                          "value": forms["bitplane"]["value"], "unit": "stimulus-cycles/s", "n_gpus": 1,
+                          "forms": forms, "roofline": rl}), flush=True)
+        return
     # the plain per-mtask graph form: heuristic and learned partitions on
     # the first cycles of the same stimuli
     gc = 8
@@ -706,14 +734,15 @@ def bench_net5(args):
                                gref, np.arange(nchk))
         learned.update(form="graph", cycles=gc, heuristic_mtasks=res["heuristic"][2],
                        learned_mtasks=res["learned"][2])
-    ns = 4
+    ns = args.cpu_sample or th
+    cap = 600
     t0 = time.perf_counter()
-    oracle.netlist_run(tb, seed, ns, budgets=np.minimum(budgets[:ns], 400), threads=th)
+    oracle.netlist_run(tb, seed, ns, budgets=np.minimum(budgets[:ns], cap), threads=th)
     cdt = time.perf_counter() - t0
-    csc = int(np.minimum(budgets[:ns], 400).astype(np.int64).sum())
+    csc = int(np.minimum(budgets[:ns], cap).astype(np.int64).sum())
     cpu = {"value": csc / cdt, "unit": "stimulus-cycles/s", "cores": min(th, ns), "kind": "port",
            "nproc": os.cpu_count(),
-           "sample": f"{ns} stimuli, budgets capped at 400 ({csc} stimulus-cycles, {cdt:.1f} s), C oracle"}
+           "sample": f"{ns} stimuli, budgets capped at {cap} ({csc} stimulus-cycles, {cdt:.1f} s), C oracle"}
     f = forms["bitplane"]
     line = {
         "metric": "stimulus-cycles/sec, bit-exact vs Verilator CPU; batched+learned vs plain GPU",

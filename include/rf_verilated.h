@@ -151,6 +151,14 @@ int rf_set_mem_images(rf_model* m, uint32_t mem, uint32_t nstim, uint32_t nelem,
  * *bytes receives the size; dst may be NULL to query it. */
 int rf_lane_state(rf_model* m, uint32_t lane, void* dst, size_t* bytes);
 
+/* Per-cycle state of one lane for a waveform dump: runs ``ncycles`` cycles
+ * of the protocol in ``mode``, one launch (or graph replay) per cycle, and
+ * after each copies lane ``lane``'s raw state (the rf_lane_state layout) on
+ * the device; dst receives ncycles records of that size. The role of the
+ * reference's VCD tracing of a model (src/V3EmitC.cpp:3927-3961,
+ * include/verilated_vcd_c.cpp), for one lane of the batch. */
+int rf_trace_lane(rf_model* m, uint32_t lane, uint32_t ncycles, uint32_t mode, void* dst, size_t bytes);
+
 /* Checkpoint / resume of the protocol form: the whole simulation state of
  * a model (every SoA slot, the lane-major memories, the per-lane change,
  * done, hash, cycle-counter and stimulus-id arrays, the device flags and

@@ -304,7 +304,7 @@ def _prologue(comp: Compiled, gen: Gen, lines: List[str], check_done=True, check
         lines.append("    if (a.done[lane]) return;")
     if check_change:
         lines.append("    if (!a.change[lane]) return;")
-    lines.append("    const size_t NP = a.npad;")
+    lines.append("    size_t NP = a.npad;")
     lines.append("    uint8_t* __restrict__ sC = a.c + lane;")
     lines.append("    uint16_t* __restrict__ sS = a.s + lane;")
     lines.append("    uint32_t* __restrict__ sI = a.i + lane;")
@@ -430,7 +430,7 @@ def body_header(comp: Compiled, name: str, ret: str = "void", extra: str = "") -
 
 
 def _body_prologue(comp: Compiled, lines: List[str]):
-    lines.append("    const size_t NP = a.npad;")
+    lines.append("    size_t NP = a.npad;")
     lines.append("    uint8_t* __restrict__ sC = a.c + lane;")
     lines.append("    uint16_t* __restrict__ sS = a.s + lane;")
     lines.append("    uint32_t* __restrict__ sI = a.i + lane;")
@@ -1144,6 +1144,7 @@ def emit_fused(comp: Compiled) -> str:
     gen.emit("}")
     gen.emit("a.hash[lane] = h;")
     gen.emit("a.cyc[lane] = cyc;")
+    _opaque_bases(gen)
     for v in sorted(gen.dirty | (gen.local & comp.stored), key=lambda x: x.id):
         if v in comp.stored:
             gen.store(v)
@@ -1154,6 +1155,15 @@ def emit_fused(comp: Compiled) -> str:
     lines += gen.top_lines + gen.lines
     lines += _lds_copy_lines(comp, gen.lds, to_lds=False) + ["}"]
     return "\n".join(lines)
+
+
+def _opaque_bases(gen: Gen):
+    """Before a kernel's final stores: the slot base pointers and stride
+    through an optimisation barrier (RF_OPAQUE_*), so the stores recompute
+    their addresses instead of reusing the prologue's across the loop."""
+    gen.emit("RF_OPAQUE_S(NP);")
+    for b in ("sC", "sS", "sI", "sQ"):
+        gen.emit(f"RF_OPAQUE_V({b});")
 
 
 def emit_reset(comp: Compiled) -> str:

@@ -24,6 +24,19 @@
 #define RF_HD static inline
 #endif
 
+// An optimisation barrier on a value: after it the compiler cannot reuse
+// values derived from the old one. The whole-cycle kernels put one on the
+// lane's slot base pointers and stride before their final stores, so the
+// slot addresses of the prologue's loads are recomputed there instead of
+// being kept live (and spilled) across the cycle loop.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define RF_OPAQUE_S(x) asm volatile("" : "+s"(x))
+#define RF_OPAQUE_V(x) asm volatile("" : "+v"(x))
+#else
+#define RF_OPAQUE_S(x) asm volatile("" : "+r"(x))
+#define RF_OPAQUE_V(x) asm volatile("" : "+r"(x))
+#endif
+
 // ---------------------------------------------------------------- masks
 RF_HD uint32_t rf_m32(int w) { return w >= 32 ? 0xffffffffu : ((1u << w) - 1u); }
 RF_HD uint64_t rf_m64(int w) { return w >= 64 ? ~0ull : ((1ull << w) - 1ull); }
@@ -302,6 +315,18 @@ RF_HD void rfw_putw(uint32_t* o, int wo, uint64_t lsb, const uint32_t* v, int wv
         const int take = (wv - k) < 32 ? (wv - k) : 32;
         rfw_put64(o, wo, lsb + (uint64_t)k, take, v[k >> 5] & rf_m32(take));
     }
+}
+// o = {o[w-1-k:0], v} in place (0 < k <= 64, v clean): a shift register
+RF_HD void rfw_shl_in(uint32_t* o, int w, int k, uint64_t v) {
+    const int n = rfw_words(w), ws = k >> 5, bs = k & 31;
+    for (int i = n - 1; i >= 0; --i) {
+        const uint32_t hi = i - ws >= 0 ? o[i - ws] : 0u;
+        const uint32_t lo = i - ws - 1 >= 0 ? o[i - ws - 1] : 0u;
+        o[i] = bs ? (hi << bs) | (lo >> (32 - bs)) : hi;
+    }
+    o[0] |= (uint32_t)v;
+    if (n > 1 && k > 32) o[1] |= (uint32_t)(v >> 32);
+    rfw_clean(o, w);
 }
 // o (width wr) = a[lsb +: wr]
 RF_HD void rfw_extract(uint32_t* o, int wr, const uint32_t* a, int wa, uint64_t lsb) {

@@ -1181,6 +1181,56 @@ int rf_lane_state(rf_model* m, uint32_t lane, void* dst, size_t* bytes) {
     return RF_OK;
 }
 
+int rf_trace_lane(rf_model* m, uint32_t lane, uint32_t ncycles, uint32_t mode, void* dst, size_t bytes) {
+    if (!m || !dst) return fail(RF_E_ARG, "null model or buffer");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
+    if (lane >= m->nlanes) return fail(RF_E_ARG, "lane %u out of range", lane);
+    if (mode > RF_MODE_BITPLANE) return fail(RF_E_ARG, "unknown run mode %u", mode);
+    size_t sb = m->d->mem_row_bytes;
+    for (int c = 0; c < 4; ++c) sb += m->d->slots[c] * kClassBytes[c];
+    if (bytes < (size_t)ncycles * sb)
+        return fail(RF_E_ARG, "trace buffer too small (%zu < %zu)", bytes, (size_t)ncycles * sb);
+    if (!ncycles) return RF_OK;
+    uint8_t* tb = nullptr;
+    RF_HIP(hipMalloc(&tb, (size_t)ncycles * sb));
+    int rc = RF_OK;
+    for (uint32_t c = 0; c < ncycles && rc == RF_OK; ++c) {
+        // one launch (or graph replay) per cycle, then the lane's slots
+        // copied out of the lane-minor arrays on the same stream
+        const uint32_t chunk = m->cfg.fused_chunk;
+        m->cfg.fused_chunk = 1;
+        rc = rf_run_cycles(m, 1, mode);
+        m->cfg.fused_chunk = chunk;
+        uint8_t* o = tb + (size_t)c * sb;
+        for (int k = 0; k < 4 && rc == RF_OK; ++k) {
+            const size_t es = kClassBytes[k];
+            if (!m->d->slots[k]) continue;
+            const hipError_t e = hipMemcpy2DAsync(o, es, (uint8_t*)m->cls[k] + lane * es, m->npad * es, es,
+                                                  m->d->slots[k], hipMemcpyDeviceToDevice, m->stream);
+            if (e != hipSuccess) rc = fail(RF_E_HIP, "trace copy: %s", hipGetErrorString(e));
+            o += m->d->slots[k] * es;
+        }
+        for (uint32_t k = 0; k < m->d->nmems && rc == RF_OK; ++k) {
+            const rf_mem& md = m->d->mems[k];
+            if (md.layout != RF_MEM_LANE_MAJOR) continue;
+            const hipError_t e = hipMemcpyAsync(o + md.loc, m->mem + md.loc * m->npad + (size_t)lane * md.row_bytes,
+                                                md.row_bytes, hipMemcpyDeviceToDevice, m->stream);
+            if (e != hipSuccess) rc = fail(RF_E_HIP, "trace copy: %s", hipGetErrorString(e));
+        }
+    }
+    if (rc == RF_OK) rc = settle(m);
+    if (rc == RF_OK) {
+        const hipError_t e = hipMemcpy(dst, tb, (size_t)ncycles * sb, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(RF_E_HIP, "trace download: %s", hipGetErrorString(e));
+    }
+    (void)hipStreamSynchronize(m->stream);
+    (void)hipFree(tb);
+    return rc;
+}
+
 int rf_profile_mtasks(rf_model* m, uint32_t reps, float* ms_out) {
     if (!m || !ms_out || !reps) return fail(RF_E_ARG, "bad argument");
     {

@@ -368,13 +368,21 @@ class Gen:
         raise CompileError(f"operator {op} not supported")
 
     # --------------------------------------------------- wide expressions
-    def exw(self, e: Expr) -> str:
-        """Name of a uint32_t array holding the (clean) wide value of ``e``."""
+    def exw(self, e: Expr, dirty_ok: bool = False) -> str:
+        """Name of a uint32_t array holding the (clean) wide value of ``e``.
+        A word-aligned slice of another wide value is a view into that
+        value's words instead of a copy, when it is clean (its top bit is a
+        word boundary or the source's top) or ``dirty_ok`` (the consumer
+        masks the top word itself)."""
         w = e.width
         n = words_of(w)
         if isinstance(e, Ref):
             self.ensure(e.var)
             return self.read_wide(e.var)
+        if isinstance(e, Sel) and isinstance(e.lsb, int) and e.lsb % 32 == 0 and is_wide(e.a.width) \
+                and e.lsb + w <= e.a.width and (dirty_ok or w % 32 == 0 or e.lsb + w == e.a.width):
+            a = self.exw(e.a, dirty_ok=True)
+            return f"({a} + {e.lsb // 32})" if e.lsb else a
         t = self.tmp("w")
         if isinstance(e, Const):
             ws = ", ".join(f"0x{(e.value >> (32 * i)) & 0xffffffff:x}u" for i in range(n))
@@ -400,7 +408,7 @@ class Gen:
             for p in e.parts:
                 pos -= p.width
                 if is_wide(p.width):
-                    self.emit(f"rfw_putw({t}, {w}, {pos}, {self.exw(p)}, {p.width});")
+                    self.emit(f"rfw_putw({t}, {w}, {pos}, {self.exw(p, dirty_ok=True)}, {p.width});")
                 else:
                     self.emit(f"rfw_put64({t}, {w}, {pos}, {p.width}, (uint64_t)({self.ex(p)}));")
             return t
@@ -461,7 +469,12 @@ class Gen:
         dst = self.target(v)
         in_nb = v in self.nb
         if lhs.whole:
-            if is_wide(v.width):
+            shift_in = self._shift_in(v, s.rhs) if is_wide(v.width) and not in_nb else None
+            if shift_in is not None:
+                k, low = shift_in
+                self.ensure(v)
+                self.emit(f"rfw_shl_in({dst}, {v.width}, {k}, (uint64_t)({low}));")
+            elif is_wide(v.width):
                 src = self.exw(s.rhs)
                 self.emit(f"rfw_copy({dst}, {src}, {v.width});")
             elif in_nb:
@@ -489,6 +502,22 @@ class Gen:
             # (a later partial write must not reload the stale stored copy)
             self.avail.add(v)
             self._write_through(v)
+
+    def _shift_in(self, v: Var, rhs: Expr):
+        """``v = {v[W-1-k:0], low}`` with ``low`` at most 64 bits: the
+        shift register idiom, done in place -> (k, low's expression)."""
+        if not isinstance(rhs, Concat) or len(rhs.parts) < 2:
+            return None
+        head = rhs.parts[0]
+        rest = rhs.parts[1:]
+        k = sum(p.width for p in rest)
+        if not (isinstance(head, Sel) and isinstance(head.a, Ref) and head.a.var is v and head.lsb == 0
+                and head.width == v.width - k and 0 < k <= 64):
+            return None
+        if any(is_wide(p.width) for p in rest):
+            return None
+        low = rest[0] if len(rest) == 1 else Concat(rest)
+        return k, self.ex(low)
 
     def _write_through(self, v: Var):
         """In per-mtask kernels, store a stored variable as soon as it is

@@ -74,6 +74,7 @@ def runtime() -> C.CDLL:
         "rf_submit_queue_static": (C.c_int, [P, u64, P, u32, P, P]),
         "rf_set_mem_images": (C.c_int, [P, u32, u32, u32, P]),
         "rf_lane_state": (C.c_int, [P, u32, P, C.POINTER(C.c_size_t)]),
+        "rf_trace_lane": (C.c_int, [P, u32, u32, u32, P, C.c_size_t]),
         "rf_checkpoint_bytes": (C.c_int, [P, C.POINTER(C.c_size_t)]),
         "rf_checkpoint_save": (C.c_int, [P, P, C.c_size_t]),
         "rf_checkpoint_load": (C.c_int, [P, P, C.c_size_t]),
@@ -251,6 +252,16 @@ class Model:
         buf = (C.c_uint8 * n.value)()
         _check(self.rt.rf_lane_state(self.h, lane, buf, C.byref(n)))
         return bytes(buf)
+
+    def trace(self, lane: int, ncycles: int, mode: int = MODE_FUSED) -> List[bytes]:
+        """Raw state of ``lane`` after each of the next ``ncycles`` cycles."""
+        n = C.c_size_t(0)
+        _check(self.rt.rf_lane_state(self.h, lane, None, C.byref(n)))
+        sb = n.value
+        buf = (C.c_uint8 * max(1, sb * ncycles))()
+        _check(self.rt.rf_trace_lane(self.h, lane, ncycles, mode, buf, sb * ncycles))
+        raw = bytes(buf)
+        return [raw[k * sb:(k + 1) * sb] for k in range(ncycles)]
 
     # ------------------------------------------------------- checkpoints
     def checkpoint_bytes(self) -> int:

@@ -53,6 +53,33 @@ def main():
 
     jobs.append(("rv32i", "batched", rv_measure,
                  lambda smp: oracle.rv32i_run(progs[:len(smp)], seed, rb[:len(smp)])))
+    # configuration 5 at full size, the gate-net engine's per-mtask graph form
+    if "learned" in registry.gatenet_meta("net5")["schedules"]:
+        from rtlflow_amd.gatenet import GateNet
+        seed5 = registry.SEED_CFG[5]
+        b5 = np.minimum(rv32i.budgets(seed5, 4096, median=2000, lo=200, hi=20000), 8)
+        tb5 = registry.gatenet_table("net5")
+
+        class _Sched:
+            def __init__(self, tag):
+                self.tag = tag
+                self.manifest = {"mtasks": registry.gatenet_meta("net5")["schedules"][tag]["mtasks"],
+                                 "partition": tag}
+
+        def gn_measure(s):
+            import time
+            with GateNet(registry.gatenet_program("net5", s.tag), 4096) as g:
+                g.run(GateNet.GRAPH, seed5, np.minimum(b5, 1))
+                t0 = time.perf_counter()
+                h, _ = g.run(GateNet.GRAPH, seed5, b5)
+                dt = time.perf_counter() - t0
+            return float(b5.astype(np.int64).sum()) / dt, h
+
+        v = judge.judge(_Sched("heuristic"), _Sched("learned"), gn_measure,
+                        lambda smp: oracle.netlist_run(tb5, seed5, len(smp), budgets=b5[:len(smp)]), sample)
+        judge.record("net5", "graph", v)
+        print(f"net5 graph: learned/heuristic {v['speedup']:.3f} bit-exact {v['bit_exact']} -> {v['choice']}",
+              flush=True)
     for name, form, measure, checker in jobs:
         if f"{name}_learned" not in registry.load_index():
             continue

@@ -102,3 +102,31 @@ def test_nonblocking_swap_group():
         hd = HostDesign(Compiled(elaborate(src), Options(threads=thr)))
         for mode in (0, 1, 2):
             assert list(hd.protocol(4, 9, 20, mode=mode, order_seed=2)) == ref
+
+
+def test_wide_slice_views_and_shift_in_idiom():
+    """Word-aligned slices of wide values read in place (compare, reduce,
+    concatenate), and ``v <= {v[W-1-k:0], x}`` shifted in place, including a
+    rotate whose shifted-in bits read v itself: host build of every form
+    against the IR interpreter."""
+    import ir_interp
+    from rtlflow_amd.compiler import Compiled, Options
+    from rtlflow_amd.hostsim import HostDesign
+    from rtlflow_amd.verilog import elaborate
+    src = """module wv(input clk, input [63:0] a, input [63:0] b, output [7:0] f, output [95:0] o);
+        reg [255:0] c; reg [199:0] d; reg [127:0] r; reg [95:0] q;
+        always @(posedge clk) begin
+            c <= {c[247:0], a[7:0] ^ b[7:0]};
+            d <= {d[159:0], a[39:0]};
+            r <= {r[126:0], r[127]};
+            q <= {c[159:128], c[63:0]} ^ {r[31:0], d[191:160], r[127:96]};
+            if (r == 128'd0) r <= {a, b};
+        end
+        assign f = {4'd0, c[255:128] < c[127:0], ^c[255:64], c[127:96] == d[127:96], c[95:64] > c[63:32]};
+        assign o = q;
+    endmodule"""
+    seed, n, cycles = 0x31D3, 4, 90
+    want = ir_interp.protocol(elaborate(src), seed, n, cycles)
+    hd = HostDesign(Compiled(elaborate(src), Options(threads=2)))
+    for mode in (0, 1, 3):
+        assert list(hd.protocol(n, seed, cycles, mode=mode)) == want, mode
