@@ -16,7 +16,9 @@ two sides of the workgroup-slot boundary.
 
 Beside it, each measured once on a bounded number of cycles that does not
 depend on ``--steps``: the fused per-lane kernel, the plain per-mtask hipGraph
-form, and the learned-vs-heuristic partition in the graph form. The CPU
+form, the learned-vs-heuristic partition in the graph form, and the
+bit-plane launch with its input words streamed from HBM (then with the
+outputs captured too), checked against the generator run. The CPU
 oracle is timed on a bounded sample as the reported CPU baseline. Optional
 legs are skipped once ``--budget-s`` seconds have passed.
 """
@@ -113,6 +115,8 @@ def main():
                     help="cycles of the fused-form comparison (0: skip)")
     ap.add_argument("--graph-cycles", type=int, default=64,
                     help="cycles of the plain graph-form comparison (0: skip)")
+    ap.add_argument("--stream-cycles", type=int, default=256,
+                    help="cycles of the input-stream leg (0: skip)")
     ap.add_argument("--learned-cycles", type=int, default=64,
                     help="cycles of the learned-vs-heuristic partition comparison (0: skip)")
     ap.add_argument("--workload", default="netlist",
@@ -242,6 +246,14 @@ def bench_netlist(args, torch):
                        "checked_stimuli": fn, "cycles": cyc}
         log(f"[bench] form={form} {fdt:.3f}s {lanes * cyc / fdt:.4g} stim-cycles/s oracle_match={fok}")
 
+    # ---- user input stream: the same launch shape with every input word
+    # read from HBM instead of the generator, then with the outputs captured
+    # as well; the stream holds the generator's own words, so the hashes must
+    # equal the generator run's for every stimulus
+    stream = None
+    if args.stream_cycles and budget_left("input stream"):
+        stream = bench_stream(args, torch, cd, man, lanes, nt, seed)
+
     # ---- learned partition: the plain graph form partitioned by the cost
     # model's choice of granularity against the instruction-count heuristic
     learned = None
@@ -329,6 +341,7 @@ def bench_netlist(args, torch):
                    "bitplane_threads": nt, "mtasks": man["mtasks"]},
         "forms": forms,
         "learned_partition": learned,
+        "input_stream": stream,
         "bit_exact_sample": all(v["bit_exact_sample"] for v in forms.values()),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -340,6 +353,78 @@ def bench_netlist(args, torch):
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def generator_words_dev(torch, man, seed, nstim, c, device="cuda"):
+    """The generator's input words of cycle ``c`` for stimuli [0, nstim) as
+    an int64 [words, nstim] device tensor: SplitMix64 of seed ^ s * golden ^
+    c << 8 ^ key (int64 arithmetic wraps like uint64; right shifts are made
+    logical by masking)."""
+    def lsr(z, k):
+        return (z >> k) & ((1 << (64 - k)) - 1)
+
+    def i64(v):
+        v &= (1 << 64) - 1
+        return v - (1 << 64) if v >= 1 << 63 else v
+
+    gold = i64(0x9E3779B97F4A7C15)
+    keys = [(p["stim_index"] << 3) | j for p in man["ports"] if p["dir"] == 0
+            for j in range((p["width"] + 63) // 64)]
+    s = torch.arange(nstim, dtype=torch.int64, device=device) * gold
+    k = torch.tensor([i64(seed ^ (c << 8) ^ key) for key in keys], dtype=torch.int64, device=device)
+    z = (s[None, :] ^ k[:, None]) + gold
+    z = (z ^ lsr(z, 30)) * i64(0xBF58476D1CE4E5B9)
+    z = (z ^ lsr(z, 27)) * i64(0x94D049BB133111EB)
+    return z ^ lsr(z, 31)
+
+
+def bench_stream(args, torch, cd, man, lanes, nt, seed):
+    from rtlflow_amd.model import MODE_BITPLANE
+    sc, reps = args.stream_cycles, 3
+    nin, nout = man["streams"]["in_words"], man["streams"]["out_words"]
+    xin = torch.empty((sc, nin, lanes), dtype=torch.int64, device="cuda")
+    for c in range(sc):
+        xin[c] = generator_words_dev(torch, man, seed, lanes, c)
+    cap = torch.zeros((sc, nout, lanes), dtype=torch.int64, device="cuda")
+    out = {"cycles": sc, "in_words": nin, "out_words": nout}
+    with cd.model(lanes, fused_chunk=args.bp_chunk, bitplane_threads=nt) as m:
+        # every run starts from the set-up state (device checkpoint)
+        m.begin(seed)
+        m.run_cycles(0, MODE_BITPLANE)
+        m.sync()
+        nb = m.checkpoint_bytes()
+        ck = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        m.checkpoint_to(ck.data_ptr(), nb)
+        m.run_cycles(sc, MODE_BITPLANE)
+        ref = m.hashes()
+        for tag, use_cap in (("stream", False), ("stream_capture", True)):
+            m.set_input_stream(xin)
+            m.set_output_capture(cap if use_cap else None)
+            m.sync()
+            m.reset_stats()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                m.restore_from(ck.data_ptr(), nb)
+                m.begin(0)
+                m.run_cycles(sc, MODE_BITPLANE)
+            m.sync()
+            dt = time.perf_counter() - t0
+            st = m.stats()
+            same = bool((m.hashes() == ref).all())
+            per_ms = st["bitplane_ms"] / max(1, st["bitplane_launches"])
+            b = 2 * ((man["bitplane"]["state_bits"] + 7) // 8) + 8 * nin + (8 * nout if use_cap else 0)
+            out[tag] = {"value": reps * lanes * sc / dt, "kernel_value": lanes * sc / (per_ms * 1e-3),
+                        "avg_launch_ms": per_ms, "alg_bytes_per_stimulus_cycle": b,
+                        "hbm_frac": b * lanes * sc / (per_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+                        "hashes_equal_generator_run": same}
+            log(f"[bench] bitplane {tag}: {out[tag]['value']:.4g} stim-cycles/s "
+                f"(kernel {out[tag]['kernel_value']:.4g}, {per_ms:.2f} ms per launch), "
+                f"hashes equal the generator run: {same}")
+        m.set_input_stream(None)
+        m.set_output_capture(None)
+    del xin, cap
+    torch.cuda.empty_cache()
+    return out
 
 
 # memory bytes a stimulus-cycle touches by design (T_mem): ALU two register

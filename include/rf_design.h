@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define RF_DESIGN_ABI 5
+#define RF_DESIGN_ABI 6
 
 /* storage classes (include/rf_heavy.h:11-17: CData/SData/IData/QData) */
 enum { RF_CLS_C = 0, RF_CLS_S = 1, RF_CLS_I = 2, RF_CLS_Q = 3 };
@@ -39,6 +39,7 @@ enum {
     RF_FLAG_UNCONVERGED = 1,  /* a lane hit the settle limit */
     RF_FLAG_QUEUE = 2,        /* batched form: next stimulus to hand out */
     RF_FLAG_ACTIVE = 3,       /* batched form: lanes still running */
+    RF_FLAG_STREAM = 4,       /* a running lane read outside the input stream */
     RF_FLAG_WORDS = 8
 };
 
@@ -69,6 +70,22 @@ typedef struct rf_mtask {
     uint32_t cost;          /* estimated instruction count */
 } rf_mtask;
 
+/* A device buffer of 64-bit words laid out [cycle][word][stimulus]: row
+ * ``c - cycle0`` holds, for every port word ``w`` in the design's port order,
+ * ``nstim`` consecutive stimuli from ``stim0``. As the input stream it
+ * replaces the SplitMix64 stimulus of the clock-cycle protocol (``words`` =
+ * the design's input words: data inputs in declaration order, 64 bits per
+ * word, masked to the port width); as the output capture it receives every
+ * output word at the point the cycle's hash is taken. Null data: unused. */
+typedef struct rf_stream {
+    uint64_t* data;
+    uint64_t stim0;
+    uint32_t nstim;
+    uint32_t cycle0;
+    uint32_t ncycles;
+    uint32_t words;
+} rf_stream;
+
 typedef struct rf_args {
     uint8_t* c;
     uint16_t* s;
@@ -93,6 +110,8 @@ typedef struct rf_args {
     uint32_t phase;      /* eval-end: RF_PHASE_* */
     uint32_t converge_limit;
     uint32_t pad0;
+    rf_stream in;        /* user stimulus (data null: SplitMix64) */
+    rf_stream out;       /* output capture (data null: none) */
 } rf_args;
 
 typedef struct rf_design {

@@ -145,6 +145,34 @@ int rf_submit_queue_bitplane(rf_model* m, uint64_t seed, const uint32_t* budgets
  * src[stimulus][nelem][words]. Pass src = NULL to clear. */
 int rf_set_mem_images(rf_model* m, uint32_t mem, uint32_t nstim, uint32_t nelem, const void* src);
 
+/* User stimulus and output capture for the clock-cycle protocol (every run
+ * mode and the batched forms). The testbench side of the reference writes
+ * each input through RTLflow::get into managed memory from the host
+ * (src/V3EmitC.cpp:5339-5350) and reads outputs back the same way; here the
+ * host hands over device buffers once and the kernels read and write them.
+ *
+ * Layout of both (rf_stream in rf_design.h): 64-bit words
+ * [cycle - cycle0][word][stimulus - stim0], nstim stimuli per row.
+ * rf_stream_words gives the words per stimulus-cycle: the input stream
+ * holds every data input (clocks excluded) in declaration order, 64 bits per
+ * word (port p's chunk j; masked to the port width when applied); the
+ * capture holds every output likewise, taken at the point each cycle's hash
+ * is folded. The design manifest lists the words ("streams").
+ *
+ * rf_set_input_stream replaces the SplitMix64 stimulus by ``words`` (device
+ * memory of the model's GPU: hipMalloc or managed; NULL restores the
+ * generator). A running lane that needs a stimulus-cycle outside the stream
+ * is reported as RF_E_ARG by the next call that waits for the device (the
+ * batched forms check their stimuli and budgets before launching).
+ * rf_set_output_capture stores the outputs of the stimulus-cycles it covers
+ * (partial windows are fine: the rest is not stored); NULL turns it off.
+ * The buffers must stay valid while runs use them. */
+int rf_stream_words(const rf_design* d, uint32_t* in_words, uint32_t* out_words);
+int rf_set_input_stream(rf_model* m, const uint64_t* words, uint64_t stim0, uint32_t nstim,
+                        uint32_t cycle0, uint32_t ncycles);
+int rf_set_output_capture(rf_model* m, uint64_t* words, uint64_t stim0, uint32_t nstim,
+                          uint32_t cycle0, uint32_t ncycles);
+
 /* Raw state of one lane, every slot in class order C, S, I, Q then the
  * lane-major memory region (for lane-k state diffs against a CPU model,
  * the role of the reference's checkN asserter, src/V3EmitC.cpp:4380-4536).

@@ -273,11 +273,15 @@ def class_bytes(w):
 
 
 def protocol(design: ir.Design, seed: int, nstim: int, cycles: int, stim0: int = 0,
-             images=None, budgets=None) -> List[int]:
+             images=None, budgets=None, inputs=None, capture=None) -> List[int]:
     """Per-stimulus FNV-1a-64 of the outputs under the clock-cycle protocol.
     ``images``: {memory name: per-stimulus initial contents, indexed by
     stimulus id}, loaded before the initial blocks run; ``budgets``: cycles
-    per stimulus (indexed from stim0) instead of ``cycles``."""
+    per stimulus (indexed from stim0) instead of ``cycles``. ``inputs``: a
+    [cycle][word][stimulus - stim0] array of 64-bit input words (data inputs
+    in declaration order, 64 bits per word) used instead of the generator;
+    ``capture``: an array of the same layout over the output words, filled
+    with every cycle's outputs."""
     data_inputs = [v for v in design.vars if v.kind == "input" and v not in design.clocks]
     outputs = [v for v in design.vars if v.kind == "output"]
     mems = {v.name: v for v in design.vars if v.is_mem}
@@ -291,16 +295,24 @@ def protocol(design: ir.Design, seed: int, nstim: int, cycles: int, stim0: int =
         h = 0xcbf29ce484222325
         ncyc = cycles if budgets is None else int(budgets[s - stim0])
         for c in range(ncyc):
+            n = 0
             for p, var in enumerate(data_inputs):
                 val = 0
                 for j in range((var.width + 63) // 64):
-                    val |= _stim(seed, s, c, (p << 3) | j) << (64 * j)
+                    w = _stim(seed, s, c, (p << 3) | j) if inputs is None else int(inputs[c][n][s - stim0])
+                    val |= w << (64 * j)
+                    n += 1
                 lane.v[var] = val & _mask(var.width)
             for ph in (0, 1):
                 for clk in design.clocks:
                     lane.v[clk] = ph
                 lane.eval()
+            n = 0
             for var in outputs:
                 h = _fnv(h, lane.v[var].to_bytes(class_bytes(var.width), "little"))
+                if capture is not None:
+                    for j in range((var.width + 63) // 64):
+                        capture[c][n][s - stim0] = (lane.v[var] >> (64 * j)) & 0xFFFFFFFFFFFFFFFF
+                        n += 1
         hashes.append(h)
     return hashes

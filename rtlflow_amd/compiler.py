@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import hashlib
 import json
-from typing import Callable, Dict, List, Optional, Set
+from typing import Callable, Dict, List, Optional, Set, Tuple
 
 from .emit_hip import ARR, CTY, Gen, cty, edge_cond, is_wide, lit
 from .instrcount import process_cost
@@ -25,7 +25,8 @@ CLS_ID = {"C": 0, "S": 1, "I": 2, "Q": 3}
 
 class Options:
     def __init__(self, threads: int = 8, block: int = BLOCK, lane_major_min_depth: int = 64,
-                 cost_fn: Optional[Callable] = None, settle_unroll: int = 2,
+                 cost_fn: Optional[Callable] = None, merge_fn: Optional[Callable] = None,
+                 settle_unroll: int = 2,
                  converge_limit: int = 100, state_in_regs: bool = True, keep=None,
                  mtask_stats: bool = False, inline_limit: int = 6000, split_always: bool = True,
                  mtask_waves: int = 4, bitplane: bool = True):
@@ -33,6 +34,7 @@ class Options:
         self.block = block
         self.lane_major_min_depth = lane_major_min_depth
         self.cost_fn = cost_fn            # vertex -> cost; default V3InstrCount restatement
+        self.merge_fn = merge_fn          # compiled design -> merge model of the contraction
         self.settle_unroll = settle_unroll
         self.converge_limit = converge_limit
         self.state_in_regs = state_in_regs
@@ -80,15 +82,19 @@ class Compiled:
             self.chglast[v] = s
         # costs
         self.vcost: List[int] = []
+        merged = self.opts.merge_fn(self) if self.opts.merge_fn is not None else None
         for vx in sch.vertices:
-            if self.opts.cost_fn is not None:
+            if merged is not None:
+                c = merged.vertex_cost(vx.id)
+            elif self.opts.cost_fn is not None:
                 c = int(self.opts.cost_fn(self, vx))
             else:
                 c = sum(process_cost(p) for p in vx.procs)
             vx.cost = max(1, c)
             self.vcost.append(vx.cost)
         edges = [(vx.id, s) for vx in sch.vertices for s in vx.succs]
-        self.mtasks: List[MTask] = partition(self.vcost, edges, self.opts.threads, order=sch.topo)
+        self.mtasks: List[MTask] = partition(self.vcost, edges, self.opts.threads, order=sch.topo,
+                                             merged=merged)
         self.vx2mt: Dict[int, int] = {}
         for k, mt in enumerate(self.mtasks):
             for vid in mt.vids:
@@ -216,7 +222,20 @@ class Compiled:
             "block": self.opts.block,
             "form": "calls" if self.calls else "inline",
             "bitplane": self.bitplane_info,
+            "streams": self.stream_table(),
         }
+
+    def stream_table(self) -> dict:
+        """Word layout of the input stream and the output capture: one
+        [port name, 64-bit chunk] pair per word, in word order."""
+        ins, outs = [], []
+        for p in self.port_table():
+            k = (p["width"] + 63) // 64
+            if p["dir"] == 0:
+                ins += [[p["name"], j] for j in range(k)]
+            elif p["hashed"]:
+                outs += [[p["name"], j] for j in range(k)]
+        return {"in_words": len(ins), "out_words": len(outs), "in": ins, "out": outs}
 
     def mem_table(self) -> List[dict]:
         out = []
@@ -495,8 +514,8 @@ def emit_bodies(comp: Compiled) -> List[str]:
     gen = Gen(comp)
     gen.top_lines = []
     gen.local = set(d.outputs)
-    _hash_code(comp, gen, "h")
-    out.append(_wrap_body(comp, gen, "rf_body_hash", "uint64_t", extra=", uint64_t h",
+    _hash_code(comp, gen, "h", "sid", "cyc")
+    out.append(_wrap_body(comp, gen, "rf_body_hash", "uint64_t", extra=", uint64_t h, uint64_t sid, uint64_t cyc",
                           tail=["    return h;"]))
     # initial blocks + clock latch
     gen = Gen(comp)
@@ -562,7 +581,7 @@ def emit_calls_kernels(comp: Compiled) -> List[str]:
              "            rf_body_setclk(a, lane, ph);"]
     lines += _calls_eval(comp, "            ")
     lines += ["        }",
-              "        h = rf_body_hash(a, lane, h);",
+              "        h = rf_body_hash(a, lane, h, sid, (uint64_t)cyc);",
               "    }",
               "    a.hash[lane] = h;",
               "    a.cyc[lane] = cyc;",
@@ -595,7 +614,7 @@ def emit_calls_kernels(comp: Compiled) -> List[str]:
               "                rf_body_setclk(a, lane, ph);"]
     lines += _calls_eval(comp, "                ")
     lines += ["            }",
-              "            h = rf_body_hash(a, lane, h);",
+              "            h = rf_body_hash(a, lane, h, (uint64_t)sid, (uint64_t)cyc);",
               "        }",
               "        a.out_hash[sid] = h;",
               f"        q = {atomic(comp, 'add', '&a.flags[RF_FLAG_QUEUE]', '1u')};",
@@ -621,7 +640,11 @@ def _change_code(comp: Compiled, gen: Gen, result: str):
         gen.dirty.add(old)
 
 
-def _hash_code(comp: Compiled, gen: Gen, h: str):
+def _hash_code(comp: Compiled, gen: Gen, h: str, sid: Optional[str] = None, cyc: Optional[str] = None):
+    """Fold the outputs into hash ``h``; with ``sid``/``cyc``, also store
+    each output's 64-bit words into the output capture (rf_args.out)."""
+    cap = []
+    n = 0
     for port in comp.port_table():
         if not port["hashed"]:
             continue
@@ -631,21 +654,33 @@ def _hash_code(comp: Compiled, gen: Gen, h: str):
         if is_wide(v.width):
             for w in range(v.words):
                 gen.emit(f"{h} = rf_fnv_bytes({h}, (uint64_t){gen.name(v)}[{w}], 4);")
+            for j in range((v.width + 63) // 64):
+                hi = (f" | ((uint64_t){gen.name(v)}[{2 * j + 1}] << 32)" if 2 * j + 1 < v.words else "")
+                cap.append(f"rf_capture(a, {sid}, {cyc}, {n}u, (uint64_t){gen.name(v)}[{2 * j}]{hi});")
+                n += 1
         else:
             gen.emit(f"{h} = rf_fnv_bytes({h}, (uint64_t){gen.get(v)}, {CLASS_BYTES[c]});")
+            cap.append(f"rf_capture(a, {sid}, {cyc}, {n}u, (uint64_t){gen.get(v)});")
+            n += 1
+    if sid is not None and cap:
+        gen.emit("if (a.out.data) {")
+        for ln in cap:
+            gen.emit("    " + ln)
+        gen.emit("}")
 
 
 def _stim_code(comp: Compiled, gen: Gen, sid: str, cyc: str):
+    n = 0
     for port in comp.port_table():
         if port["dir"] != 0:
             continue
         v = port["var"]
         gen.declare(v)
         p = port["stim_index"]
-        n = v.name
         if is_wide(v.width):
             for j in range((v.width + 63) // 64):
-                gen.emit(f"{{ const uint64_t r = rf_stim(a.seed, {sid}, {cyc}, {(p << 3) | j}u);")
+                gen.emit(f"{{ const uint64_t r = rf_input(a, {sid}, {cyc}, {(p << 3) | j}u, {n}u);")
+                n += 1
                 gen.emit(f"  {gen.name(v)}[{2 * j}] = (uint32_t)r;")
                 if 2 * j + 1 < v.words:
                     gen.emit(f"  {gen.name(v)}[{2 * j + 1}] = (uint32_t)(r >> 32); }}")
@@ -653,8 +688,9 @@ def _stim_code(comp: Compiled, gen: Gen, sid: str, cyc: str):
                     gen.emit("}")
             gen.emit(f"rfw_clean({gen.name(v)}, {v.width});")
         else:
-            gen.emit(gen.set(v, f"({cty(v.width)})(rf_stim(a.seed, {sid}, {cyc}, {p << 3}u) & "
+            gen.emit(gen.set(v, f"({cty(v.width)})(rf_input(a, {sid}, {cyc}, {p << 3}u, {n}u) & "
                                 f"{lit((1 << v.width) - 1, 64)})"))
+            n += 1
         gen.avail.add(v)
         gen.dirty.add(v)
 
@@ -689,7 +725,7 @@ def emit_eval_end(comp: Compiled) -> str:
     gen.emit("} else if (a.phase == RF_PHASE_HASH) {")
     gen.ind += 1
     gen.emit("uint64_t h = a.hash[lane];")
-    _hash_code(comp, gen, "h")
+    _hash_code(comp, gen, "h", "(uint64_t)a.stim[lane]", "(uint64_t)a.cyc[lane]")
     gen.emit("a.hash[lane] = h;")
     gen.emit("a.cyc[lane] += 1u;")
     gen.ind -= 1
@@ -1139,7 +1175,7 @@ def emit_fused(comp: Compiled) -> str:
     _eval_loop_body_split(comp, gen, "n == 0u", cone)
     gen.ind -= 1
     gen.emit("}")
-    _hash_code(comp, gen, "h")
+    _hash_code(comp, gen, "h", "sid", "(uint64_t)cyc")
     gen.ind -= 1
     gen.emit("}")
     gen.emit("a.hash[lane] = h;")

@@ -187,12 +187,15 @@ __device__ __forceinline__ void bp_store(const rf_args& a, uint32_t cls, uint64_
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
 // Inputs of one cycle into their planes (wavefront 0): set A, read by the
-// clk = 0 settle.
+// clk = 0 settle. Chunk k of the table is input word k (the generator's or
+// the input stream's).
+// ``live``: the lane runs that cycle (inactive lanes and lanes past their
+// budget still fill their plane bits, which nothing reads).
 template <typename TP>
-__device__ __forceinline__ void bp_inputs(uint64_t* L, TP ic, uint32_t n_in, uint64_t seed, uint64_t sid,
-                                          uint32_t cyc, uint32_t t) {
+__device__ __forceinline__ void bp_inputs(uint64_t* L, TP ic, uint32_t n_in, const rf_args& a, uint64_t sid,
+                                          uint32_t cyc, uint32_t t, bool live) {
     for (uint32_t k = 0; k < n_in; ++k, ic += 4) {
-        const uint64_t r = rf_stim(seed, sid, (uint64_t)cyc, ic[0]);
+        const uint64_t r = rf_input(a, sid, (uint64_t)cyc, ic[0], k, live);
         bp_to_planes(L, ic[2], r, ic[1], t);
     }
 }
@@ -206,18 +209,36 @@ __device__ __forceinline__ void bp_copy_inputs(uint64_t* L, TP ic, uint32_t n_in
 }
 
 // Fold this cycle's outputs into the lane's hash (wavefront 1); ``hp``: the
-// output-slot table of program M or B.
+// output-slot table of program M or B. With ``cap`` the outputs' 64-bit
+// words also go to the output capture as stimulus ``sid``'s cycle ``c``.
 template <typename TP>
-__device__ __forceinline__ uint64_t bp_hash(const uint64_t* L, TP hp, uint32_t n_hash, uint64_t h, uint32_t t) {
+__device__ __forceinline__ uint64_t bp_hash(const uint64_t* L, TP hp, uint32_t n_hash, uint64_t h, uint32_t t,
+                                            const rf_args& a, bool cap, uint64_t sid, uint64_t c) {
+    uint32_t n = 0;
     for (uint32_t k = 0; k < n_hash; ++k) {
         const uint32_t nbytes = hp[0], width = hp[1];
         const TP refs = hp + 2;
         if (width <= 64) {
-            h = rf_fnv_bytes(h, bp_gather(L, refs, width, t), (int)nbytes);
+            const uint64_t v = bp_gather(L, refs, width, t);
+            h = rf_fnv_bytes(h, v, (int)nbytes);
+            if (cap) rf_capture(a, sid, c, n, v);
+            ++n;
         } else {
+            uint64_t lo = 0;
             for (uint32_t w = 0; 4u * w < nbytes; ++w) {
                 const uint32_t nb = min(32u, width - 32u * w);
-                h = rf_fnv_bytes(h, bp_gather(L, refs + 32u * w, nb, t), 4);
+                const uint64_t v = bp_gather(L, refs + 32u * w, nb, t);
+                h = rf_fnv_bytes(h, v, 4);
+                if (w & 1u) {
+                    if (cap) rf_capture(a, sid, c, n, lo | (v << 32));
+                    ++n;
+                } else {
+                    lo = v;
+                }
+            }
+            if ((nbytes >> 2) & 1u) {
+                if (cap) rf_capture(a, sid, c, n, lo);
+                ++n;
             }
         }
         hp += 2 + width;
@@ -436,7 +457,7 @@ rf_bp_kernel(rf_bp_params P) {
         }
     }
     const uint32_t* in_tab = prog + P.off_in;
-    if (w0 && a.ncycles) bp_inputs(L, in_tab, P.n_in, a.seed, sid, cyc, t);
+    if (w0 && a.ncycles) bp_inputs(L, in_tab, P.n_in, a, sid, cyc, t, act && lim > 0);
     // per-cycle tables into LDS after the planes
     const lds_u32* tc = reinterpret_cast<const lds_u32*>((size_t)P.tab_off);
     const lds_u32* th = tc + P.n_commit[1];
@@ -481,7 +502,7 @@ rf_bp_kernel(rf_bp_params P) {
         if (P.n_mem) bp_mem_ports(L, P, 0, wave, NW, t, lane, inb, act);
         if (w0) {
             bp_copy_inputs(L, in_tab, P.n_in, t);
-            if (N > 1) bp_inputs(L, in_tab, P.n_in, a.seed, sid, cyc + 1, t);
+            if (N > 1) bp_inputs(L, in_tab, P.n_in, a, sid, cyc + 1, t, act && lim > 1);
         }
         __syncthreads();
         // cycle n's outputs and cycle n + 1's next state in one pass
@@ -496,16 +517,17 @@ rf_bp_kernel(rf_bp_params P) {
                 bp_commit<NT>(L, prog + P.off_commit[1], P.n_commit[1], tid);
             if (P.n_mem) bp_mem_ports(L, P, 1, wave, NW, t, lane, inb, act);
             if (w1) {
-                const uint64_t hn = P.tab_off ? bp_hash(L, th, P.n_hash, h, t)
-                                              : bp_hash(L, prog + P.off_hash[0], P.n_hash, h, t);
+                const bool cap = a.out.data && act && n < lim;
+                const uint64_t hn = P.tab_off ? bp_hash(L, th, P.n_hash, h, t, a, cap, sid, cyc + n)
+                                              : bp_hash(L, prog + P.off_hash[0], P.n_hash, h, t, a, cap, sid, cyc + n);
                 if (act && n < lim) h = hn;
             } else if (w0) {
                 if (P.tab_off) {
                     bp_copy_inputs(L, ti, P.n_in, t);
-                    if (n + 2 < N) bp_inputs(L, ti, P.n_in, a.seed, sid, cyc + n + 2, t);
+                    if (n + 2 < N) bp_inputs(L, ti, P.n_in, a, sid, cyc + n + 2, t, act && n + 2 < lim);
                 } else {
                     bp_copy_inputs(L, in_tab, P.n_in, t);
-                    if (n + 2 < N) bp_inputs(L, in_tab, P.n_in, a.seed, sid, cyc + n + 2, t);
+                    if (n + 2 < N) bp_inputs(L, in_tab, P.n_in, a, sid, cyc + n + 2, t, act && n + 2 < lim);
                 }
             }
             __syncthreads();
@@ -513,7 +535,8 @@ rf_bp_kernel(rf_bp_params P) {
         // the last cycle's outputs
         bp_phase<NT, BA>(L, lev[RF_BP_B], gates[RF_BP_B], P.nlev[RF_BP_B], tid);
         if (w1) {
-            const uint64_t hn = bp_hash(L, prog + P.off_hash[1], P.n_hash, h, t);
+            const bool cap = a.out.data && act && N - 1 < lim;
+            const uint64_t hn = bp_hash(L, prog + P.off_hash[1], P.n_hash, h, t, a, cap, sid, cyc + N - 1);
             if (act && N - 1 < lim) h = hn;
         }
         __syncthreads();

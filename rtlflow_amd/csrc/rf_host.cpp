@@ -23,6 +23,7 @@ typedef void (*host_kernel)(rf_args, uint32_t);
 
 uint32_t g_state_lane = ~0u;
 uint8_t* g_state_out = nullptr;
+rf_stream g_in{}, g_out{};  // input stream / output capture of later runs
 
 struct HostModel {
     const rf_design* d;
@@ -61,6 +62,8 @@ struct HostModel {
         a.npad = npad;
         a.nlanes = n;
         a.converge_limit = 100;
+        a.in = g_in;
+        a.out = g_out;
     }
     void all(const void* k, uint32_t phase) {
         rf_args x = a;
@@ -197,6 +200,7 @@ RF_EXPORT int rf_host_protocol(const rf_design* d, uint32_t nlanes, uint64_t see
         }
     }
     if (m.flags[RF_FLAG_UNCONVERGED]) return -4;
+    if (m.flags[RF_FLAG_STREAM]) return -1;
     memcpy(hashes, m.hash.data(), nlanes * 8ull);
     if (g_state_lane < nlanes && g_state_out) {
         // raw state of one lane in the rf_lane_state order: C, S, I, Q slots,
@@ -222,6 +226,13 @@ RF_EXPORT void rf_host_capture(uint32_t lane, uint8_t* out) {
     g_state_out = out;
 }
 
+// Input stream and output capture (host memory here) for the runs that
+// follow; NULL: the generator / no capture.
+RF_EXPORT void rf_host_streams(const rf_stream* in, const rf_stream* out) {
+    g_in = in ? *in : rf_stream{};
+    g_out = out ? *out : rf_stream{};
+}
+
 RF_EXPORT int rf_host_queue(const rf_design* d, uint32_t nlanes, uint64_t seed, const uint32_t* budgets,
                             uint32_t nstim, const uint32_t* order, uint64_t* out,
                             const uint64_t* images) {
@@ -238,7 +249,7 @@ RF_EXPORT int rf_host_queue(const rf_design* d, uint32_t nlanes, uint64_t seed, 
     x.nstim = nstim;
     host_kernel f = (host_kernel)d->k_persistent;
     for (uint32_t l = 0; l < nlanes; ++l) f(x, l);
-    return m.flags[RF_FLAG_UNCONVERGED] ? -4 : 0;
+    return m.flags[RF_FLAG_UNCONVERGED] ? -4 : m.flags[RF_FLAG_STREAM] ? -1 : 0;
 }
 
 }  // extern "C"

@@ -426,6 +426,71 @@ int check_flags(rf_model* m) {
     if (f[RF_FLAG_UNCONVERGED])
         return fail(RF_E_CONVERGE, "Verilated model didn't converge (settle limit %u reached)",
                     m->cfg.converge_limit);
+    if (f[RF_FLAG_STREAM]) {
+        const rf_stream& in = m->base.in;
+        return fail(RF_E_ARG, "a running lane read a stimulus-cycle outside the input stream "
+                    "(stimuli [%llu, %llu), cycles [%u, %llu))", (unsigned long long)in.stim0,
+                    (unsigned long long)(in.stim0 + in.nstim), in.cycle0,
+                    (unsigned long long)in.cycle0 + in.ncycles);
+    }
+    return RF_OK;
+}
+
+// 64-bit words per stimulus-cycle of the input stream and the output capture.
+void stream_words(const rf_design* d, uint32_t* nin, uint32_t* nout) {
+    uint32_t a = 0, b = 0;
+    for (uint32_t p = 0; p < d->nports; ++p) {
+        const rf_port& q = d->ports[p];
+        const uint32_t k = (q.width + 63) / 64;
+        if (q.dir == RF_PORT_IN) a += k;
+        else if (q.dir == RF_PORT_OUT && q.hashed) b += k;
+    }
+    *nin = a;
+    *nout = b;
+}
+
+int set_stream(rf_model* m, rf_stream* dst, uint64_t* words, uint64_t stim0, uint32_t nstim, uint32_t cycle0,
+               uint32_t ncycles, uint32_t nwords, const char* what) {
+    if (!words) {
+        *dst = rf_stream{};
+        m->graphs_stale = true;
+        return RF_OK;
+    }
+    if (!nstim || !ncycles) return fail(RF_E_ARG, "%s: empty window (%u stimuli, %u cycles)", what, nstim, ncycles);
+    if (((uintptr_t)words & 7u) != 0) return fail(RF_E_ARG, "%s: buffer not 8-byte aligned", what);
+    hipPointerAttribute_t at{};
+    const hipError_t e = hipPointerGetAttributes(&at, words);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(RF_E_ARG, "%s: %p is not GPU memory", what, (void*)words);
+    }
+    const bool dev = at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged ||
+                     at.type == hipMemoryTypeUnified;
+    if (!dev) return fail(RF_E_ARG, "%s: %p is not device or managed memory", what, (void*)words);
+    if (at.type == hipMemoryTypeDevice && at.device != m->device)
+        return fail(RF_E_ARG, "%s: buffer on GPU %d, model on GPU %d", what, at.device, m->device);
+    // drain queued launches that may still read the old buffer's settings
+    // (they captured their arguments already; this only keeps the order of
+    // errors plain)
+    const int src = settle(m);
+    if (src) return src;
+    *dst = rf_stream{words, stim0, nstim, cycle0, ncycles, nwords};
+    m->graphs_stale = true;
+    return RF_OK;
+}
+
+// The batched forms hand out stimulus ids [0, nstim) for cycles [0, budget):
+// the input stream must hold all of them.
+int queue_stream_check(rf_model* m, const uint32_t* budgets, uint32_t nstim) {
+    const rf_stream& in = m->base.in;
+    if (!in.data) return RF_OK;
+    uint32_t mx = 0;
+    for (uint32_t k = 0; k < nstim; ++k) mx = std::max(mx, budgets[k]);
+    if (in.stim0 != 0 || in.nstim < nstim || (mx && (in.cycle0 != 0 || in.ncycles < mx)))
+        return fail(RF_E_ARG, "the input stream holds stimuli [%llu, %llu) and cycles [%u, %llu); "
+                    "the queue needs stimuli [0, %u) and cycles [0, %u)", (unsigned long long)in.stim0,
+                    (unsigned long long)(in.stim0 + in.nstim), in.cycle0,
+                    (unsigned long long)in.cycle0 + in.ncycles, nstim, mx);
     return RF_OK;
 }
 
@@ -804,6 +869,29 @@ int rf_hashes(rf_model* m, uint32_t lane0, uint32_t count, uint64_t* out) {
     return RF_OK;
 }
 
+int rf_stream_words(const rf_design* d, uint32_t* in_words, uint32_t* out_words) {
+    if (!d || !in_words || !out_words) return fail(RF_E_ARG, "null argument");
+    stream_words(d, in_words, out_words);
+    return RF_OK;
+}
+
+int rf_set_input_stream(rf_model* m, const uint64_t* words, uint64_t stim0, uint32_t nstim, uint32_t cycle0,
+                        uint32_t ncycles) {
+    if (!m) return fail(RF_E_ARG, "null model");
+    uint32_t nin, nout;
+    stream_words(m->d, &nin, &nout);
+    return set_stream(m, &m->base.in, const_cast<uint64_t*>(words), stim0, nstim, cycle0, ncycles, nin,
+                      "rf_set_input_stream");
+}
+
+int rf_set_output_capture(rf_model* m, uint64_t* words, uint64_t stim0, uint32_t nstim, uint32_t cycle0,
+                          uint32_t ncycles) {
+    if (!m) return fail(RF_E_ARG, "null model");
+    uint32_t nin, nout;
+    stream_words(m->d, &nin, &nout);
+    return set_stream(m, &m->base.out, words, stim0, nstim, cycle0, ncycles, nout, "rf_set_output_capture");
+}
+
 int rf_set_mem_images(rf_model* m, uint32_t mi, uint32_t nstim, uint32_t nelem, const void* src) {
     if (!m) return fail(RF_E_ARG, "null model");
     {
@@ -906,8 +994,10 @@ int rf_submit_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_
         if (src) return src;
     }
     if (!nstim) return RF_OK;
+    int qrc = queue_stream_check(m, budgets, nstim);
+    if (qrc) return qrc;
     std::vector<uint32_t> ord;
-    int qrc = queue_setup(m, budgets, nstim, order, ord);
+    qrc = queue_setup(m, budgets, nstim, order, ord);
     if (qrc) return qrc;
     return run_queue(m, seed, budgets, nstim, ord.data(), m->budget, m->order, m->out_hash, out_hashes);
 }
@@ -926,6 +1016,10 @@ int rf_submit_queue_static(rf_model* m, uint64_t seed, const uint32_t* budgets, 
         if (src) return src;
     }
     if (!nstim) return RF_OK;
+    {
+        const int qrc = queue_stream_check(m, budgets, nstim);
+        if (qrc) return qrc;
+    }
     std::lock_guard<std::mutex> lk(g_static_order_mu);
     uint32_t* ord = g_static_order;
     if (order) {
@@ -962,9 +1056,12 @@ int rf_submit_queue_bitplane(rf_model* m, uint64_t seed, const uint32_t* budgets
     if (m->base.images)
         return fail(RF_E_STATE, "memory images are loaded by the per-lane batched form only");
     if (!nstim) return RF_OK;
-    std::vector<uint32_t> ord;
-    int rc = queue_setup(m, budgets, nstim, order, ord);
+    int rc = queue_stream_check(m, budgets, nstim);
     if (rc) return rc;
+    std::vector<uint32_t> ord;
+    rc = queue_setup(m, budgets, nstim, order, ord);
+    if (rc) return rc;
+    RF_HIP(hipMemsetAsync(m->flags, 0, RF_FLAG_WORDS * 4, m->stream));
     RF_HIP(hipMemcpyAsync(m->budget, budgets, (size_t)nstim * 4, hipMemcpyHostToDevice, m->stream));
     RF_HIP(hipMemsetAsync(m->out_hash, 0, (size_t)nstim * 8, m->stream));
     const uint32_t nl = m->nlanes;
@@ -1017,7 +1114,7 @@ int rf_submit_queue_bitplane(rf_model* m, uint64_t seed, const uint32_t* budgets
     RF_HIP(hipMemcpyAsync(m->done, done.data(), m->npad, hipMemcpyHostToDevice, m->stream));
     RF_HIP(hipStreamSynchronize(m->stream));
     for (uint32_t k = 0; k < nstim; ++k) out_hashes[k] = budgets[k] ? oh[k] : 0xcbf29ce484222325ull;
-    return RF_OK;
+    return check_flags(m);
 }
 
 // ---------------------------------------------------------------- checkpoints

@@ -14,8 +14,19 @@
 // 0x100000001b3) over the little-endian bytes of each output port's storage
 // words (1/2/4/8 bytes for the u8/u16/u32/u64 classes, 4 per word for wide
 // ports), output ports in declaration order, once per clock cycle.
+//
+// A user input stream (rf_args.in) replaces the generator: word n of the
+// cycle's input words (port p's chunk j, ports in the same order) is read
+// from the stream instead of r(seed, s, c, (p << 3) | j). The output capture
+// (rf_args.out) stores each output port's 64-bit chunks where the hash is
+// folded. Reads outside the stream's window give 0 and captures outside the
+// capture's window are dropped. A running lane that reads outside the input
+// stream gets 0 and raises RF_FLAG_STREAM, which the runtime reports as an
+// error.
 #pragma once
 #include <stdint.h>
+
+#include "rf_design.h"
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
@@ -45,4 +56,30 @@ RF_HD uint64_t rf_fnv_bytes(uint64_t h, uint64_t v, int nbytes) {
         h *= RF_FNV_PRIME;
     }
     return h;
+}
+
+// A running lane asked for an input the stream does not hold.
+RF_HD void rf_stream_miss(const rf_args& a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicOr(&a.flags[RF_FLAG_STREAM], 1u);
+#else
+    a.flags[RF_FLAG_STREAM] |= 1u;
+#endif
+}
+
+// Input word n (generator key k) of stimulus s in cycle c. ``live``: the
+// lane runs this cycle (a miss is then an error the runtime reports).
+RF_HD uint64_t rf_input(const rf_args& a, uint64_t s, uint64_t c, uint64_t k, uint32_t n, bool live = true) {
+    if (!a.in.data) return rf_stim(a.seed, s, c, k);
+    const uint64_t ds = s - a.in.stim0, dc = c - a.in.cycle0;
+    if (ds < a.in.nstim && dc < a.in.ncycles) return a.in.data[(dc * a.in.words + n) * a.in.nstim + ds];
+    if (live) rf_stream_miss(a);
+    return 0;
+}
+
+// Output word n of stimulus s in cycle c into the capture, if it covers them.
+RF_HD void rf_capture(const rf_args& a, uint64_t s, uint64_t c, uint32_t n, uint64_t v) {
+    const uint64_t ds = s - a.out.stim0, dc = c - a.out.cycle0;
+    if (a.out.data && ds < a.out.nstim && dc < a.out.ncycles)
+        a.out.data[(dc * a.out.words + n) * a.out.nstim + ds] = v;
 }

@@ -115,6 +115,15 @@ def emit_vtop_header(manifest: dict) -> str:
     o.append("    void run_cycles(uint32_t n, uint32_t mode = RF_MODE_FUSED) {")
     o.append("        check(rf_run_cycles(m_, n, mode));")
     o.append("    }")
+    o.append("    // user stimulus / output capture: device buffers [cycle][word][stimulus]")
+    o.append("    void set_input_stream(const uint64_t* dev, uint32_t nstim, uint32_t ncycles,")
+    o.append("                          uint64_t stim0 = 0, uint32_t cycle0 = 0) {")
+    o.append("        check(rf_set_input_stream(m_, dev, stim0, nstim, cycle0, ncycles));")
+    o.append("    }")
+    o.append("    void set_output_capture(uint64_t* dev, uint32_t nstim, uint32_t ncycles,")
+    o.append("                            uint64_t stim0 = 0, uint32_t cycle0 = 0) {")
+    o.append("        check(rf_set_output_capture(m_, dev, stim0, nstim, cycle0, ncycles));")
+    o.append("    }")
     o.append("    std::vector<uint64_t> hashes() {")
     o.append("        std::vector<uint64_t> h(lanes());")
     o.append("        check(rf_hashes(m_, 0, (uint32_t)h.size(), h.data()));")

@@ -50,8 +50,32 @@ class HostDesign:
         self.lib.rf_host_protocol.argtypes = [P, u32, u64, u32, u32, u32, u64, P, P]
         self.lib.rf_host_queue.restype = C.c_int
         self.lib.rf_host_queue.argtypes = [P, u32, u64, P, u32, P, P, P]
+        self.lib.rf_host_streams.restype = None
+        self.lib.rf_host_streams.argtypes = [P, P]
+        self._streams = None
         self.comp = comp
         self.mems = comp.mem_table()
+
+    def set_streams(self, inputs=None, capture=None, in_stim0: int = 0, in_cycle0: int = 0,
+                    cap_stim0: int = 0, cap_cycle0: int = 0):
+        """Input stream / output capture (numpy uint64 [cycles, words,
+        stimuli], host memory) for the runs that follow; None: generator / no
+        capture."""
+        from .model import rf_stream
+        nin = self.comp.stream_table()["in_words"]
+        nout = self.comp.stream_table()["out_words"]
+
+        def mk(arr, s0, c0, words):
+            if arr is None:
+                return None
+            assert arr.dtype == np.uint64 and arr.flags.c_contiguous and arr.shape[1] == words
+            return rf_stream(arr.ctypes.data, s0, arr.shape[2], c0, arr.shape[0], words)
+
+        si = mk(inputs, in_stim0, in_cycle0, nin)
+        so = mk(capture, cap_stim0, cap_cycle0, nout)
+        self._streams = (si, so, inputs, capture)
+        self.lib.rf_host_streams(None if si is None else C.addressof(si),
+                                 None if so is None else C.addressof(so))
 
     def _images(self, images):
         """{memory name: array [nstim, nelem]} -> (table, keep-alive list)."""

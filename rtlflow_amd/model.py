@@ -13,7 +13,7 @@ from __future__ import annotations
 import ctypes as C
 import json
 from pathlib import Path
-from typing import Dict, List, Optional, Sequence, Union
+from typing import Dict, List, Optional, Sequence, Tuple, Union
 
 import numpy as np
 
@@ -40,6 +40,13 @@ class rf_stats(C.Structure):
     _fields_ = [("fused_launches", C.c_uint64), ("fused_ms", C.c_double),
                 ("graph_launches", C.c_uint64), ("graph_ms", C.c_double),
                 ("bitplane_launches", C.c_uint64), ("bitplane_ms", C.c_double)]
+
+
+class rf_stream(C.Structure):
+    """A [cycle][word][stimulus] buffer of 64-bit words (input stream or
+    output capture)."""
+    _fields_ = [("data", C.c_void_p), ("stim0", C.c_uint64), ("nstim", C.c_uint32),
+                ("cycle0", C.c_uint32), ("ncycles", C.c_uint32), ("words", C.c_uint32)]
 
 
 _rt = None
@@ -73,6 +80,9 @@ def runtime() -> C.CDLL:
         "rf_submit_queue_bitplane": (C.c_int, [P, u64, P, u32, P, P]),
         "rf_submit_queue_static": (C.c_int, [P, u64, P, u32, P, P]),
         "rf_set_mem_images": (C.c_int, [P, u32, u32, u32, P]),
+        "rf_stream_words": (C.c_int, [P, C.POINTER(u32), C.POINTER(u32)]),
+        "rf_set_input_stream": (C.c_int, [P, P, u64, u32, u32, u32]),
+        "rf_set_output_capture": (C.c_int, [P, P, u64, u32, u32, u32]),
         "rf_lane_state": (C.c_int, [P, u32, P, C.POINTER(C.c_size_t)]),
         "rf_trace_lane": (C.c_int, [P, u32, u32, u32, P, C.c_size_t]),
         "rf_checkpoint_bytes": (C.c_int, [P, C.POINTER(C.c_size_t)]),
@@ -252,6 +262,47 @@ class Model:
         buf = (C.c_uint8 * n.value)()
         _check(self.rt.rf_lane_state(self.h, lane, buf, C.byref(n)))
         return bytes(buf)
+
+    # ------------------------------------------------------- streams
+    def stream_words(self) -> Tuple[int, int]:
+        """64-bit words per stimulus-cycle of the input stream and of the
+        output capture."""
+        a, b = C.c_uint32(0), C.c_uint32(0)
+        _check(self.rt.rf_stream_words(self.design.desc, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    @staticmethod
+    def _window(buf, words: int, what: str) -> Tuple[int, int, int]:
+        """(pointer, ncycles, nstim) of a device buffer shaped [ncycles,
+        words, nstim] (a torch tensor of 64-bit integers on this GPU)."""
+        shape = tuple(buf.shape)
+        if len(shape) != 3 or shape[1] != words:
+            raise RFError(f"{what}: expected shape [cycles, {words}, stimuli], got {list(shape)}")
+        if buf.element_size() != 8 or not buf.is_contiguous():
+            raise RFError(f"{what}: expected a contiguous tensor of 64-bit words")
+        return buf.data_ptr(), shape[0], shape[2]
+
+    def set_input_stream(self, buf, stim0: int = 0, cycle0: int = 0):
+        """Inputs from ``buf`` [cycles, in_words, stimuli] (device memory)
+        instead of the generator; None restores the generator."""
+        if buf is None:
+            _check(self.rt.rf_set_input_stream(self.h, None, 0, 0, 0, 0))
+            self._in_buf = None
+            return
+        ptr, nc, ns = self._window(buf, self.stream_words()[0], "input stream")
+        _check(self.rt.rf_set_input_stream(self.h, ptr, stim0, ns, cycle0, nc))
+        self._in_buf = buf
+
+    def set_output_capture(self, buf, stim0: int = 0, cycle0: int = 0):
+        """Store every output word of the covered stimulus-cycles into ``buf``
+        [cycles, out_words, stimuli] (device memory); None turns it off."""
+        if buf is None:
+            _check(self.rt.rf_set_output_capture(self.h, None, 0, 0, 0, 0))
+            self._out_buf = None
+            return
+        ptr, nc, ns = self._window(buf, self.stream_words()[1], "output capture")
+        _check(self.rt.rf_set_output_capture(self.h, ptr, stim0, ns, cycle0, nc))
+        self._out_buf = buf
 
     def trace(self, lane: int, ncycles: int, mode: int = MODE_FUSED) -> List[bytes]:
         """Raw state of ``lane`` after each of the next ``ncycles`` cycles."""

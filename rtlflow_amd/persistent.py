@@ -113,7 +113,7 @@ def emit_persistent(comp) -> str:
     _eval_loop_body_split(comp, gen, "cyc == 0u", cone)
     gen.ind -= 1
     gen.emit("}")
-    _hash_code(comp, gen, "h")
+    _hash_code(comp, gen, "h", "(uint64_t)sid", "(uint64_t)cyc")
     gen.ind -= 1
     gen.emit("}")
     gen.emit("a.out_hash[sid] = h;")

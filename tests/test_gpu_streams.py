@@ -1,0 +1,199 @@
+"""GPU: user input streams and output capture (rf_set_input_stream /
+rf_set_output_capture) in every run mode and both batched forms, against
+the IR interpreter given the same input words; at 4,096 lanes, a stream of
+the generator's own words reproduces the generator run bit for bit."""
+import numpy as np
+import pytest
+
+import ir_interp
+import oracle
+from rtlflow_amd.designs import registry
+from rtlflow_amd.designs.mix import VERILOG as MIX
+from rtlflow_amd.model import MODE_BITPLANE, MODE_FUSED, MODE_GRAPH, RFError
+from rtlflow_amd.verilog import elaborate
+
+SEED = registry.SEED_CFG[2]
+GOLD = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _splitmix(x):
+    with np.errstate(over="ignore"):
+        z = x + GOLD
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def _keys(man):
+    """Generator key of every input word, in stream order."""
+    return [(p["stim_index"] << 3) | j for p in man["ports"] if p["dir"] == 0
+            for j in range((p["width"] + 63) // 64)]
+
+
+def _generator_words(man, seed, nstim, ncyc, stim0=0, cycle0=0):
+    s = np.arange(stim0, stim0 + nstim, dtype=np.uint64)
+    keys = _keys(man)
+    out = np.empty((ncyc, len(keys), nstim), np.uint64)
+    with np.errstate(over="ignore"):
+        for c in range(ncyc):
+            for n, k in enumerate(keys):
+                out[c, n] = _splitmix(np.uint64(seed) ^ (s * GOLD) ^ np.uint64((cycle0 + c) << 8) ^ np.uint64(k))
+    return out
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int64)).cuda()
+
+
+def _host(t):
+    return t.cpu().numpy().view(np.uint64)
+
+
+def test_generator_words_restate_the_generator():
+    man = {"ports": [{"dir": 0, "stim_index": 0, "width": 7}, {"dir": 2, "stim_index": 0xffffffff, "width": 1},
+                     {"dir": 0, "stim_index": 1, "width": 70}]}
+    w = _generator_words(man, 0x1234, 3, 4, stim0=5)
+    for c in range(4):
+        for n, k in enumerate([0, 8, 9]):
+            for s in range(3):
+                assert int(w[c, n, s]) == ir_interp._stim(0x1234, 5 + s, c, k)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [MODE_GRAPH, MODE_FUSED, MODE_BITPLANE])
+def test_stream_and_capture_match_interpreter(gpu, mode):
+    """150 lanes (a partial 64-lane group), stimuli from id 5: 9 cycles from
+    the generator (graphs captured), then 21 cycles from a stream set on
+    the running model, in two runs; the capture covers every cycle."""
+    import torch
+    cd = registry.get("mix")
+    man = cd.manifest
+    nin, nout = man["streams"]["in_words"], man["streams"]["out_words"]
+    n, stim0, c1, c2 = 150, 5, 9, 21
+    rng = np.random.default_rng(100 + mode)
+    tail = rng.integers(0, 2**64, size=(c2, nin, n), dtype=np.uint64, endpoint=False)
+    full_in = np.concatenate([_generator_words(man, SEED, n, c1, stim0=stim0), tail])
+    want = np.zeros((c1 + c2, nout, n), np.uint64)
+    ref = ir_interp.protocol(elaborate(MIX), SEED, n, c1 + c2, stim0=stim0, inputs=full_in,
+                             capture=want)
+    cap = torch.zeros((c1 + c2, nout, n), dtype=torch.int64, device="cuda")
+    xin = _dev(tail)
+    with cd.model(n) as m:
+        assert m.stream_words() == (nin, nout)
+        m.begin(SEED, stim0=stim0)
+        m.set_output_capture(cap, stim0=stim0)
+        m.run_cycles(c1, mode)
+        m.set_input_stream(xin, stim0=stim0, cycle0=c1)
+        m.run_cycles(8, mode)
+        m.run_cycles(c2 - 8, mode)
+        h = m.hashes()
+    assert [int(v) for v in h] == ref
+    assert (_host(cap) == want).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["lanes", "bitplane"])
+def test_batched_queue_with_stream(gpu, form):
+    """Uneven budgets (0 included), more stimuli than lanes: the queue reads
+    each stimulus's words from the stream and captures only the cycles of its
+    own budget."""
+    cd = registry.get("mix")
+    man = cd.manifest
+    nin, nout = man["streams"]["in_words"], man["streams"]["out_words"]
+    rng = np.random.default_rng(7)
+    budgets = rng.integers(0, 25, size=140).astype(np.uint32)
+    budgets[3] = 0
+    ns, nc = len(budgets), int(budgets.max())
+    x = rng.integers(0, 2**64, size=(nc, nin, ns), dtype=np.uint64, endpoint=False)
+    want = np.zeros((nc, nout, ns), np.uint64)
+    ref = ir_interp.protocol(elaborate(MIX), 0, ns, nc, budgets=budgets, inputs=x, capture=want)
+    import torch
+    cap = torch.zeros((nc, nout, ns), dtype=torch.int64, device="cuda")
+    with cd.model(64) as m:
+        m.set_input_stream(_dev(x))
+        m.set_output_capture(cap)
+        h = m.submit_queue(0, budgets, mode=MODE_FUSED if form == "lanes" else MODE_BITPLANE)
+        assert [int(v) for v in h] == ref
+        assert (_host(cap) == want).all()
+        # a stream one cycle short of the longest budget is refused up front
+        m.set_input_stream(_dev(x[:-1]))
+        with pytest.raises(RFError, match="input stream"):
+            m.submit_queue(0, budgets)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [MODE_FUSED, MODE_BITPLANE])
+def test_generator_words_reproduce_generator_at_scale(gpu, mode):
+    """net_small at 4,096 lanes: a stream of the generator's own words gives
+    the generator run's hashes and outputs (size-independent), and both match
+    the C oracle on a strided sample."""
+    import torch
+    name, n, nc = "net_small", 4096, 40
+    cd = registry.get(name)
+    man = cd.manifest
+    nout = man["streams"]["out_words"]
+    cap_gen = torch.zeros((nc, nout, n), dtype=torch.int64, device="cuda")
+    cap_str = torch.zeros_like(cap_gen)
+    with cd.model(n) as m:
+        m.begin(SEED)
+        m.set_output_capture(cap_gen)
+        m.run_cycles(nc, mode)
+        h_gen = m.hashes()
+        m.begin(0)
+        m.set_input_stream(_dev(_generator_words(man, SEED, n, nc)))
+        m.set_output_capture(cap_str)
+        m.run_cycles(nc, mode)
+        h_str = m.hashes()
+    assert (h_gen == h_str).all()
+    assert torch.equal(cap_gen, cap_str)
+    idx = np.arange(0, n, 97)
+    ref = oracle.netlist_run(registry.table_bytes(name), SEED, nstim=n, cycles=nc)
+    assert (h_gen[idx] == ref[idx]).all()
+
+
+@pytest.mark.gpu
+def test_partial_capture_window_bitplane(gpu):
+    import torch
+    cd = registry.get("net_small")
+    nout = cd.manifest["streams"]["out_words"]
+    n, nc = 256, 20
+    full = torch.zeros((nc, nout, n), dtype=torch.int64, device="cuda")
+    part = torch.zeros((10, nout, 64), dtype=torch.int64, device="cuda")
+    with cd.model(n) as m:
+        m.begin(SEED)
+        m.set_output_capture(full)
+        m.run_cycles(nc, MODE_BITPLANE)
+        m.begin(SEED)
+        m.set_output_capture(part, stim0=70, cycle0=5)
+        m.run_cycles(nc, MODE_BITPLANE)
+    assert torch.equal(part, full[5:15, :, 70:134])
+
+
+@pytest.mark.gpu
+def test_stream_errors(gpu):
+    import torch
+    cd = registry.get("mix")
+    nin, nout = cd.manifest["streams"]["in_words"], cd.manifest["streams"]["out_words"]
+    with cd.model(64) as m:
+        with pytest.raises(RFError, match="memory"):
+            m.set_input_stream(torch.zeros((4, nin, 64), dtype=torch.int64))     # host memory
+        with pytest.raises(RFError, match="shape"):
+            m.set_input_stream(torch.zeros((4, nin + 1, 64), dtype=torch.int64, device="cuda"))
+        x = torch.zeros((4, nin, 64), dtype=torch.int64, device="cuda")
+        m.begin(1)
+        m.set_input_stream(x)
+        m.run_cycles(5, MODE_FUSED)          # cycle 4 is past the stream
+        with pytest.raises(RFError, match="outside the input stream"):
+            m.hashes()
+        m.begin(1)                           # clears the report
+        m.run_cycles(4, MODE_BITPLANE)
+        m.hashes()
+        m.set_input_stream(None)             # back to the generator
+        m.begin(SEED)
+        m.run_cycles(6, MODE_FUSED)
+        g = m.hashes()
+    with cd.model(64) as m2:
+        m2.begin(SEED)
+        m2.run_cycles(6, MODE_BITPLANE)
+        assert (m2.hashes() == g).all()
