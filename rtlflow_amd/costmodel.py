@@ -256,6 +256,204 @@ class LearnedCost:
         return self._cache[key][vx.id]
 
 
+def forward_cpu(model: Dict, raw: Sequence[Sequence[float]]) -> np.ndarray:
+    """The trained network's prediction (microseconds) on the host: the same
+    F -> 64 -> 64 -> 1 ReLU network and parameter layout as the device
+    kernels (csrc/rf_mlp.hip: W1 [64][F], b1, W2 [64][64], b2, w3, b3), in
+    FP64, for use inside the compiler where no GPU is present."""
+    p = np.asarray(model["params"], np.float64)
+    F = N_FEATURES
+    H = 64
+    o = 0
+    w1 = p[o:o + H * F].reshape(H, F); o += H * F
+    b1 = p[o:o + H]; o += H
+    w2 = p[o:o + H * H].reshape(H, H); o += H * H
+    b2 = p[o:o + H]; o += H
+    w3 = p[o:o + H]; o += H
+    b3 = p[o]
+    x = (np.log1p(np.asarray(raw, np.float64)) - np.asarray(model["mu"], np.float64)) / \
+        np.asarray(model["sd"], np.float64)
+    h1 = np.maximum(x @ w1.T + b1, 0.0)
+    h2 = np.maximum(h1 @ w2.T + b2, 0.0)
+    y = (h2 @ w3 + b3) * model["ys"] + model["ym"]
+    return np.exp(y)
+
+
+class LearnedMerge:
+    """Merge model for the contraction (``partition.contract``): the cost of
+    an mtask is the trained network's predicted kernel time of its vertex
+    set, less that of an empty kernel, in units of 1/scale us. The scores of
+    the reference's contraction (src/V3Partition.cpp:1402-1438) add the two
+    costs of a candidate merge; here the union is costed as a whole, so a
+    merge of mtasks that read the same variables (fewer distinct inputs, less
+    state touched) scores lower than the sum of its parts, which an additive
+    instruction count cannot see.
+
+    Per mtask it keeps the feature aggregates: instruction, branch and
+    wide-op counts, and bit sets (Python integers over variable indices) of
+    the variables read, written, and written-and-read-outside. A union's
+    features follow from those: only variables one side writes and the other
+    reads can turn internal, so only their readers are inspected."""
+
+    def __init__(self, model: Dict, comp, scale: float = 100.0):
+        self.scale = scale
+        p = np.asarray(model["params"], np.float64)
+        F, H = N_FEATURES, 64
+        o = 0
+        self.w1 = p[o:o + H * F].reshape(H, F); o += H * F
+        self.b1 = p[o:o + H]; o += H
+        self.w2 = p[o:o + H * H].reshape(H, H); o += H * H
+        self.b2 = p[o:o + H]; o += H
+        self.w3 = p[o:o + H]; o += H
+        self.b3 = float(p[o])
+        self.mu = np.asarray(model["mu"], np.float64)
+        self.sd = np.asarray(model["sd"], np.float64)
+        self.ym, self.ys = float(model["ym"]), float(model["ys"])
+        fx = FeatureExtractor(comp)
+        sch = comp.sch
+        stored = self._stored_like(comp)
+        idx: Dict[object, int] = {}
+        for vx in sch.vertices:
+            for v in vx.reads | vx.need_old | vx.writes:
+                if v not in idx:
+                    idx[v] = len(idx)
+        self.nbytes = [0] * len(idx)
+        for v, i in idx.items():
+            if v in stored and not v.is_mem:
+                self.nbytes[i] = v.slots * CLS_BYTES[class_of(v)]
+            elif v.is_mem:
+                self.nbytes[i] = CLS_BYTES[class_of(v)] * v.words
+        # readers of each variable index (vertex ids)
+        self.readers: List[List[int]] = [[] for _ in idx]
+        for v, rs in fx.readers.items():
+            if v in idx:
+                self.readers[idx[v]] = sorted(rs)
+        self.owner: List[int] = list(range(len(sch.vertices)))
+        self.members: Dict[int, List[int]] = {}
+        self.R: Dict[int, int] = {}
+        self.W: Dict[int, int] = {}
+        self.X: Dict[int, int] = {}     # written and read by a vertex outside
+        self.cnt: Dict[int, tuple] = {}
+        self.state: Dict[int, int] = {}
+        for vx in sch.vertices:
+            r = w = x = 0
+            for v in vx.reads | vx.need_old:
+                r |= 1 << idx[v]
+            for v in vx.writes:
+                w |= 1 << idx[v]
+                if any(rd != vx.id for rd in self.readers[idx[v]]):
+                    x |= 1 << idx[v]
+            self.R[vx.id], self.W[vx.id], self.X[vx.id] = r, w, x
+            b, wd = fx.counts[vx.id]
+            self.cnt[vx.id] = (sum(process_cost(q) for q in vx.procs), b, wd)
+            self.state[vx.id] = self._bytes(r | w)
+            self.members[vx.id] = [vx.id]
+        self.base = self._predict([0.0] * N_FEATURES)
+        self._cache: Dict[tuple, tuple] = {}
+
+    @staticmethod
+    def _stored_like(comp) -> Set:
+        """Variables the partition will keep in the SoA arrays, as far as it
+        is known before partitioning: ports, memories, state, and values
+        that more than one vertex touches (the rest are vertex-local
+        temporaries)."""
+        sch = comp.sch
+        touched: Dict[object, int] = {}
+        for vx in sch.vertices:
+            for v in vx.reads | vx.need_old | vx.writes:
+                touched[v] = touched.get(v, 0) + 1
+        out = set()
+        for v, n in touched.items():
+            if v.kind in ("input", "output") or v.is_mem or sch.roles.get(v) == "state" or n > 1:
+                out.add(v)
+        return out
+
+    def _bytes(self, bits: int) -> int:
+        total = 0
+        nb = self.nbytes
+        while bits:
+            low = bits & -bits
+            total += nb[low.bit_length() - 1]
+            bits ^= low
+        return total
+
+    def _predict(self, raw: Sequence[float]) -> float:
+        x = (np.log1p(np.asarray(raw, np.float64)) - self.mu) / self.sd
+        h1 = np.maximum(self.w1 @ x + self.b1, 0.0)
+        h2 = np.maximum(self.w2 @ h1 + self.b2, 0.0)
+        return float(np.exp((self.w3 @ h2 + self.b3) * self.ys + self.ym))
+
+    def _x_union(self, a: int, b: int) -> int:
+        """Written-and-read-outside set of the union of mtasks a and b."""
+        xa, xb = self.X[a], self.X[b]
+        out = xa | xb
+        owner = self.owner
+        for cand in ((xa & self.R[b]), (xb & self.R[a])):
+            while cand:
+                low = cand & -cand
+                cand ^= low
+                if all(owner[r] in (a, b) for r in self.readers[low.bit_length() - 1]):
+                    out &= ~low
+        return out
+
+    def _union(self, a: int, b: int) -> tuple:
+        r = self.R[a] | self.R[b]
+        w = self.W[a] | self.W[b]
+        x = self._x_union(a, b)
+        ca, cb = self.cnt[a], self.cnt[b]
+        state = self.state[a] + self.state[b] - self._bytes((self.R[a] | self.W[a]) & (self.R[b] | self.W[b]))
+        feats = [float(ca[0] + cb[0]), float(bin(r & ~w).count("1")), float(bin(x).count("1")),
+                 float(state), float(ca[1] + cb[1]), float(ca[2] + cb[2])]
+        us = self._predict(feats)
+        return max(1, int(round((us - self.base) * self.scale))), x, state
+
+    def vertex_cost(self, vid: int) -> int:
+        r, w, x = self.R[vid], self.W[vid], self.X[vid]
+        c = self.cnt[vid]
+        feats = [float(c[0]), float(bin(r & ~w).count("1")), float(bin(x).count("1")),
+                 float(self.state[vid]), float(c[1]), float(c[2])]
+        return max(1, int(round((self._predict(feats) - self.base) * self.scale)))
+
+    def _get(self, ta, tb) -> tuple:
+        key = (ta.id, ta.gen, tb.id, tb.gen) if ta.id < tb.id else (tb.id, tb.gen, ta.id, ta.gen)
+        u = self._cache.get(key)
+        if u is None:
+            u = self._union(ta.id, tb.id)
+            self._cache[key] = u
+        return u
+
+    def cost(self, ta, tb) -> int:
+        return self._get(ta, tb)[0]
+
+    def merge(self, keep: int, gone: int):
+        _, x, state = self._union(keep, gone)
+        self.X[keep] = x
+        self.state[keep] = state
+        self.R[keep] |= self.R.pop(gone)
+        self.W[keep] |= self.W.pop(gone)
+        self.X.pop(gone)
+        self.state.pop(gone)
+        ck, cg = self.cnt[keep], self.cnt.pop(gone)
+        self.cnt[keep] = (ck[0] + cg[0], ck[1] + cg[1], ck[2] + cg[2])
+        moved = self.members.pop(gone)
+        for v in moved:
+            self.owner[v] = keep
+        self.members[keep].extend(moved)
+
+
+def learned_merge(model: Optional[Dict] = None):
+    """Options.merge_fn for a compile: the committed trained model's
+    LearnedMerge of the design being compiled."""
+    if model is None:
+        import json
+        from pathlib import Path
+        model = json.loads((Path(__file__).parent / "costtables" / "model.json").read_text())
+
+    def make(comp):
+        return LearnedMerge(model, comp)
+    return make
+
+
 # =====================================================================
 # Training data, training, prediction
 # =====================================================================

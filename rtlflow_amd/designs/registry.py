@@ -101,6 +101,10 @@ def build_entry(name: str, jobs: int = 8, force: bool = False) -> dict:
     if learned:
         choice = json.loads((COSTTABLES / "partition_choice.json").read_text())
         opts["threads"] = int(choice[learned]["threads"])
+    if opts.pop("learned_merge", False):
+        # contraction scored by the learned cost of each candidate union
+        from ..costmodel import learned_merge
+        opts["merge_fn"] = learned_merge()
     table = opts.pop("cost_table", None)
     if table:
         # partition with the learned per-vertex costs instead of instruction counts
@@ -201,6 +205,13 @@ register("net2_learned", _netlist(seed=SEED_CFG[2], name="net2"), learned_thread
 register("systolic_learned", _systolic(), learned_threads="systolic", bitplane=False)
 register("net5s_learned", ENTRIES["net5s"].make, learned_threads="net5s", bitplane=False)
 register("rv32i_learned", _rv32i(), learned_threads="rv32i", bitplane=False)
+# the learned granularity with the contraction's merges scored by the
+# learned cost of each candidate union (costmodel.LearnedMerge)
+register("net2_lmerge", _netlist(seed=SEED_CFG[2], name="net2"), learned_threads="net2",
+         learned_merge=True, bitplane=False)
+register("systolic_lmerge", _systolic(), learned_threads="systolic", learned_merge=True, bitplane=False)
+register("net5s_lmerge", ENTRIES["net5s"].make, learned_threads="net5s", learned_merge=True, bitplane=False)
+register("rv32i_lmerge", _rv32i(), learned_threads="rv32i", learned_merge=True, bitplane=False)
 
 # the autotuner's choice of compile-time options, where it measured one
 # (rtlflow_amd/tuned/<name>.json, written by rtlflow_amd.autotune on the GPU)
@@ -247,6 +258,17 @@ def _rams():
 
 
 register("rams", _rams(), threads=2)
+
+
+def _mix():
+    def make():
+        from .mix import VERILOG
+        return {"verilog": VERILOG, "top": "mix"}
+    return make
+
+
+# every port shape of the input stream / output capture layouts
+register("mix", _mix(), threads=2)
 
 # combinational cycles: one that settles, one that never does
 def _settle(which: str):

@@ -1,11 +1,13 @@
 """The heuristic schedule judges the learned one.
 
-A design has two partitions: the instruction-count heuristic's (V3InstrCount
-costs, src/V3Partition.cpp:456) and the learned one (the granularity the MLP
-predicts fastest, ``*_learned`` libraries). For each form the learned
-partition is kept only if its measured throughput beats the heuristic's and
-its hashes equal both the heuristic's and an independent checker's on the
-same stimuli; otherwise the heuristic partition is used. The verdicts of a
+A design has up to three partitions: the instruction-count heuristic's
+(V3InstrCount costs, src/V3Partition.cpp:456), the learned granularity (the
+``--threads`` the MLP predicts fastest, ``*_learned`` libraries) and the
+learned granularity with the contraction's merges scored by the MLP's cost of
+each candidate union (``*_lmerge``). For each form a learned partition is
+kept only if its measured throughput beats the heuristic's and its hashes
+equal both the heuristic's and an independent checker's on the same stimuli;
+otherwise the heuristic partition is used. The verdicts of a
 GPU run are committed in ``costtables/judge.json``; ``scheduled()`` returns
 the library a form should run.
 """
@@ -101,11 +103,22 @@ def record(name: str, form: str, verdict: Dict):
     JUDGE.write_text(json.dumps(d, indent=1) + "\n")
 
 
+# learned partitions a verdict may keep: the learned granularity alone
+# (``<name>_learned``, verdict key ``form``) and with the learned merge
+# scores (``<name>_lmerge``, verdict key ``form + MERGE``)
+MERGE = "+merge"
+VARIANTS = (("", "_learned"), (MERGE, "_lmerge"))
+
+
 def scheduled(name: str, form: str):
-    """The library a form of design ``name`` runs: the learned partition
-    where the committed verdict kept it, the heuristic one otherwise."""
+    """The library a form of design ``name`` runs: of the learned partitions
+    the committed verdicts kept, the fastest; the heuristic one when none
+    was kept."""
     from .designs import registry
-    v = verdicts().get(name, {}).get(form)
-    if v and v.get("kept") and f"{name}_learned" in registry.load_index():
-        return registry.get(f"{name}_learned")
-    return registry.get(name)
+    index = registry.load_index()
+    best, lib = 1.0, name
+    for key, suffix in VARIANTS:
+        v = verdicts().get(name, {}).get(form + key)
+        if v and v.get("kept") and f"{name}{suffix}" in index and v["speedup"] > best:
+            best, lib = v["speedup"], f"{name}{suffix}"
+    return registry.get(lib)

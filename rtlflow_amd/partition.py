@@ -124,11 +124,11 @@ class Graph:
         return False
 
     # --------------------------------------------------------------- merge
-    def merge(self, keep: int, gone: int):
+    def merge(self, keep: int, gone: int, cost: Optional[int] = None):
         k = self.m[keep]
         g = self.m[gone]
         k.vids.extend(g.vids)
-        k.cost += g.cost
+        k.cost = k.cost + g.cost if cost is None else cost
         k.sc = step_cost(k.cost)
         for p in g.preds:
             if p == keep:
@@ -217,18 +217,25 @@ def _rev_without(g: Graph, ta: MTask, tb: MTask) -> int:
     return max((m[s].cp_rev + m[s].sc for s in ta.succs if s != b), default=0)
 
 
-def edge_score(g: Graph, a: int, b: int) -> int:
+def _union_cost(g: Graph, ta: MTask, tb: MTask, merged) -> int:
+    """Cost of the mtask a merge of ta and tb would make: the sum of their
+    costs (additive instruction counts), or the merge model's cost of the
+    union when one is given."""
+    return ta.cost + tb.cost if merged is None else merged.cost(ta, tb)
+
+
+def edge_score(g: Graph, a: int, b: int, merged=None) -> int:
     """Local critical path through the merged node if edge a->b merges."""
     ta, tb = g.m[a], g.m[b]
     fwd = max(ta.cp_fwd, _fwd_without(g, tb, ta))
     rev = max(tb.cp_rev, _rev_without(g, ta, tb))
-    return fwd + rev + step_cost(ta.cost + tb.cost)
+    return fwd + rev + step_cost(_union_cost(g, ta, tb, merged))
 
 
-def sibling_score(g: Graph, a: int, b: int) -> int:
+def sibling_score(g: Graph, a: int, b: int, merged=None) -> int:
     ta, tb = g.m[a], g.m[b]
     return (max(ta.cp_fwd, tb.cp_fwd) + max(ta.cp_rev, tb.cp_rev)
-            + step_cost(ta.cost + tb.cost))
+            + step_cost(_union_cost(g, ta, tb, merged)))
 
 
 SIBLING_EDGE_LIMIT = 25
@@ -237,9 +244,12 @@ SIBLING_EDGE_LIMIT = 25
 MAX_MTASKS_PER_THREAD = 50
 
 
-def contract(g: Graph, cp_limit: int, max_mtasks: int = 1 << 30):
+def contract(g: Graph, cp_limit: int, max_mtasks: int = 1 << 30, merged=None):
     """PartContraction: merge the lowest-scoring candidates while the merged
-    local critical path stays within ``cp_limit``."""
+    local critical path stays within ``cp_limit``. ``merged``: optional
+    merge model (``cost(ta, tb)`` of a union, ``merge(keep, gone)`` told of
+    every merge) whose non-additive costs replace the sum in the scores and
+    in the merged mtask's cost."""
     alive_count = len(g.alive())
     g.init_cp()
     heap: List[Tuple[int, int, int, int, int, int]] = []  # score, kind, a, b, gen_a, gen_b
@@ -247,7 +257,7 @@ def contract(g: Graph, cp_limit: int, max_mtasks: int = 1 << 30):
 
     def push_edge(a, b):
         nonlocal seq
-        heapq.heappush(heap, (edge_score(g, a, b) + 1, 1, seq, a, b, 0))
+        heapq.heappush(heap, (edge_score(g, a, b, merged) + 1, 1, seq, a, b, 0))
         seq += 1
 
     def push_sib(a, b):
@@ -256,7 +266,7 @@ def contract(g: Graph, cp_limit: int, max_mtasks: int = 1 << 30):
             return
         if a > b:
             a, b = b, a
-        heapq.heappush(heap, (sibling_score(g, a, b), 0, seq, a, b, 0))
+        heapq.heappush(heap, (sibling_score(g, a, b, merged), 0, seq, a, b, 0))
         seq += 1
 
     def sibs_of(t: MTask):
@@ -292,9 +302,9 @@ def contract(g: Graph, cp_limit: int, max_mtasks: int = 1 << 30):
         if kind == 1:
             if b not in ta.succs:
                 continue
-            actual = edge_score(g, a, b) + 1
+            actual = edge_score(g, a, b, merged) + 1
         else:
-            actual = sibling_score(g, a, b)
+            actual = sibling_score(g, a, b, merged)
         if actual > score:
             heapq.heappush(heap, (actual, kind, seq, a, b, 0))
             seq += 1
@@ -325,7 +335,12 @@ def contract(g: Graph, cp_limit: int, max_mtasks: int = 1 << 30):
             if tb.cp_fwd > ta.cp_fwd and g.has_path(b, a, skip_direct=False):
                 continue
         keep, gone = (a, b) if (len(ta.preds) + len(ta.succs)) >= (len(tb.preds) + len(tb.succs)) else (b, a)
-        g.merge(keep, gone)
+        if merged is None:
+            g.merge(keep, gone)
+        else:
+            c = merged.cost(g.m[keep], g.m[gone])
+            merged.merge(keep, gone)
+            g.merge(keep, gone, cost=c)
         alive_count -= 1
         g.update_cp_local(keep)
         tk = g.m[keep]
@@ -337,17 +352,19 @@ def contract(g: Graph, cp_limit: int, max_mtasks: int = 1 << 30):
 
 
 def partition(costs: List[int], edges: List[Tuple[int, int]], threads: int,
-              order: Optional[List[int]] = None) -> List[MTask]:
+              order: Optional[List[int]] = None, merged=None) -> List[MTask]:
     """Coarsen a vertex DAG. Returns mtasks in a topological order; each
     mtask's ``vids`` are sorted by ``order`` (the vertices' topological
-    order) so its body is a valid sequential schedule."""
+    order) so its body is a valid sequential schedule. ``merged``: optional
+    merge model for the contraction (see ``contract``); ``costs`` are then
+    its costs of the single vertices."""
     g = Graph(costs, edges)
     total = sum(costs)
     threads = max(threads, 1)
     cp_limit = (total * 3) // (threads * 5)
     if threads <= 1:
         cp_limit = 1 << 62
-    contract(g, cp_limit, MAX_MTASKS_PER_THREAD * threads if threads > 1 else 500)
+    contract(g, cp_limit, MAX_MTASKS_PER_THREAD * threads if threads > 1 else 500, merged)
     g.remove_transitive()
     pos = {v: k for k, v in enumerate(order)} if order is not None else None
     out = []

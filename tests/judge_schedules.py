@@ -81,12 +81,13 @@ def main():
         print(f"net5 graph: learned/heuristic {v['speedup']:.3f} bit-exact {v['bit_exact']} -> {v['choice']}",
               flush=True)
     for name, form, measure, checker in jobs:
-        if f"{name}_learned" not in registry.load_index():
-            continue
-        v = judge.judge(registry.get(name), registry.get(f"{name}_learned"), measure, checker, sample)
-        judge.record(name, form, v)
-        print(f"{name} {form}: learned/heuristic {v['speedup']:.3f} bit-exact {v['bit_exact']} "
-              f"-> {v['choice']}", flush=True)
+        for key, suffix in judge.VARIANTS:
+            if f"{name}{suffix}" not in registry.load_index():
+                continue
+            v = judge.judge(registry.get(name), registry.get(f"{name}{suffix}"), measure, checker, sample)
+            judge.record(name, form + key, v)
+            print(f"{name} {form + key}: {suffix[1:]}/heuristic {v['speedup']:.3f} bit-exact "
+                  f"{v['bit_exact']} -> {v['choice']}", flush=True)
 
 
 if __name__ == "__main__":

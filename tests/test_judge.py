@@ -38,3 +38,17 @@ def test_committed_verdicts_are_consistent():
         for form, v in forms.items():
             assert v["kept"] == (v["bit_exact"] and v["learned"] > v["heuristic"]
                                  and not v.get("identical", False)), (name, form)
+
+
+def test_scheduled_takes_the_fastest_kept_variant(tmp_path, monkeypatch):
+    from rtlflow_amd.designs import registry
+    monkeypatch.setattr(judge, "JUDGE", tmp_path / "judge.json")
+    monkeypatch.setattr(registry, "load_index", lambda: {"d": 1, "d_learned": 1, "d_lmerge": 1})
+    monkeypatch.setattr(registry, "get", lambda name: name)
+    judge.record("d", "graph", {"kept": True, "speedup": 1.1})
+    judge.record("d", "graph" + judge.MERGE, {"kept": True, "speedup": 1.3})
+    judge.record("d", "batched", {"kept": True, "speedup": 1.2})
+    judge.record("d", "batched" + judge.MERGE, {"kept": False, "speedup": 1.5})
+    assert judge.scheduled("d", "graph") == "d_lmerge"
+    assert judge.scheduled("d", "batched") == "d_learned"
+    assert judge.scheduled("d", "other") == "d"
