@@ -140,6 +140,26 @@ int rf_submit_queue_static(rf_model* m, uint64_t seed, const uint32_t* budgets, 
 int rf_submit_queue_bitplane(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
                              const uint32_t* order, uint64_t* out_hashes);
 
+/* The batched workload in resumable steps. rf_queue_open sets up the queue
+ * (order may be NULL: longest budget first) without running it;
+ * rf_queue_step runs the next max_stimuli queue positions to completion in
+ * ``mode`` (RF_MODE_FUSED: the persistent per-lane kernel; RF_MODE_BITPLANE:
+ * bit-plane rounds) and returns the new position; rf_queue_hashes gives
+ * every stimulus's hash so far (0 for those not yet run). Between steps no
+ * lane holds a stimulus, so the queue's whole state is its budgets, order,
+ * position and hashes: rf_queue_save writes them (rf_queue_save_bytes: the
+ * size) and rf_queue_load restores them into any model of the same design,
+ * whatever its lane count, to continue the batch -- the queue half of the
+ * reference's --savable model save/restore (src/V3EmitC.cpp:2957-3039) for
+ * long batches. Memory images and input streams are model settings: set
+ * them again on the model that resumes. Hashes equal rf_submit_queue's. */
+int rf_queue_open(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim, const uint32_t* order);
+int rf_queue_step(rf_model* m, uint32_t max_stimuli, uint32_t mode, uint32_t* position);
+int rf_queue_hashes(rf_model* m, uint64_t* out_hashes, uint32_t nstim);
+int rf_queue_save_bytes(rf_model* m, size_t* bytes);
+int rf_queue_save(rf_model* m, void* dst, size_t bytes);
+int rf_queue_load(rf_model* m, const void* src, size_t bytes);
+
 /* Per-stimulus memory images for the batched form: before a stimulus runs,
  * elements [0, nelem) of memory ``mem`` in its lane are loaded from
  * src[stimulus][nelem][words]. Pass src = NULL to clear. */

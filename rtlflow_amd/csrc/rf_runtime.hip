@@ -107,6 +107,14 @@ struct rf_model {
     };
     std::vector<pending> pend;
     std::vector<hipEvent_t> evfree;
+    // the open resumable queue (rf_queue_open): device budget / order /
+    // out_hash hold the whole queue, positions [0, qpos) have run
+    struct queue_run {
+        bool open = false;
+        uint64_t seed = 0;
+        uint32_t nstim = 0, pos = 0;
+        std::vector<uint32_t> budgets, ord;
+    } q;
 };
 
 namespace {
@@ -892,6 +900,191 @@ int rf_set_output_capture(rf_model* m, uint64_t* words, uint64_t stim0, uint32_t
     return set_stream(m, &m->base.out, words, stim0, nstim, cycle0, ncycles, nout, "rf_set_output_capture");
 }
 
+// ---- resumable queue: the batched workload run in steps of queue
+// positions; between steps no lane holds a stimulus, so the queue's state is
+// its order, budgets, position and the hashes so far (rf_queue_save)
+
+static int queue_setup(rf_model* m, const uint32_t* budgets, uint32_t nstim, const uint32_t* order,
+                       std::vector<uint32_t>& ord);
+static int bitplane_rounds(rf_model* m, uint64_t seed, const uint32_t* budgets, const uint32_t* ord,
+                           uint32_t n, uint32_t nstim);
+
+int rf_queue_open(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim, const uint32_t* order) {
+    if (!m || (!budgets && nstim)) return fail(RF_E_ARG, "null argument");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
+    m->q = rf_model::queue_run{};
+    int rc = queue_stream_check(m, budgets, nstim);
+    if (rc) return rc;
+    std::vector<uint32_t> ord;
+    rc = queue_setup(m, budgets, nstim, order, ord);
+    if (rc) return rc;
+    if (nstim) {
+        RF_HIP(hipMemcpyAsync(m->budget, budgets, (size_t)nstim * 4, hipMemcpyHostToDevice, m->stream));
+        RF_HIP(hipMemcpyAsync(m->order, ord.data(), (size_t)nstim * 4, hipMemcpyHostToDevice, m->stream));
+        RF_HIP(hipMemsetAsync(m->out_hash, 0, (size_t)nstim * 8, m->stream));
+        RF_HIP(hipStreamSynchronize(m->stream));
+    }
+    m->q.open = true;
+    m->q.seed = seed;
+    m->q.nstim = nstim;
+    m->q.budgets.assign(budgets, budgets + nstim);
+    m->q.ord = std::move(ord);
+    return RF_OK;
+}
+
+int rf_queue_step(rf_model* m, uint32_t max_stimuli, uint32_t mode, uint32_t* position) {
+    if (!m) return fail(RF_E_ARG, "null model");
+    if (!m->q.open) return fail(RF_E_STATE, "no open queue (rf_queue_open or rf_queue_load first)");
+    if (mode != RF_MODE_FUSED && mode != RF_MODE_BITPLANE)
+        return fail(RF_E_ARG, "queue steps run in RF_MODE_FUSED or RF_MODE_BITPLANE (got %u)", mode);
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
+    auto& q = m->q;
+    const uint32_t n = std::min(max_stimuli, q.nstim - q.pos);
+    int rc = RF_OK;
+    if (n && mode == RF_MODE_BITPLANE) {
+        if (!m->bp_prog) return fail(RF_E_STATE, "design %s has no bit-plane program", m->d->name);
+        if (m->base.images)
+            return fail(RF_E_STATE, "memory images are loaded by the per-lane batched form only");
+        RF_HIP(hipMemsetAsync(m->flags, 0, RF_FLAG_WORDS * 4, m->stream));
+        rc = bitplane_rounds(m, q.seed, q.budgets.data(), q.ord.data() + q.pos, n, q.nstim);
+    } else if (n) {
+        uint32_t f[RF_FLAG_WORDS] = {0};
+        f[RF_FLAG_QUEUE] = m->nlanes;
+        RF_HIP(hipMemcpyAsync(m->flags, f, sizeof f, hipMemcpyHostToDevice, m->stream));
+        rf_args a = args_for(m, RF_PHASE_EVAL);
+        a.seed = q.seed;
+        a.budget = m->budget;
+        a.order = m->order + q.pos;   // this step's queue positions
+        a.out_hash = m->out_hash;
+        a.nstim = n;
+        RF_HIP(hipEventRecord(m->ev0, m->stream));
+        rc = launch(m, m->d->k_persistent, a);
+        if (rc) return rc;
+        RF_HIP(hipEventRecord(m->ev1, m->stream));
+        RF_HIP(hipEventSynchronize(m->ev1));
+        float ms = 0;
+        RF_HIP(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+        m->stats.fused_launches++;
+        m->stats.fused_ms += ms;
+        m->did_init = true;
+        rc = check_flags(m);
+    }
+    if (rc) return rc;
+    q.pos += n;
+    if (position) *position = q.pos;
+    return RF_OK;
+}
+
+int rf_queue_hashes(rf_model* m, uint64_t* out_hashes, uint32_t nstim) {
+    if (!m || (!out_hashes && nstim)) return fail(RF_E_ARG, "null argument");
+    if (!m->q.open) return fail(RF_E_STATE, "no open queue");
+    if (nstim != m->q.nstim) return fail(RF_E_ARG, "queue holds %u stimuli, buffer %u", m->q.nstim, nstim);
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
+    if (!nstim) return RF_OK;
+    RF_HIP(hipMemcpy(out_hashes, m->out_hash, (size_t)nstim * 8, hipMemcpyDeviceToHost));
+    // stimuli that have not run read 0; those of budget 0 that have, the
+    // FNV offset basis (as rf_submit_queue returns them)
+    std::vector<uint8_t> ran(nstim, 0);
+    for (uint32_t k = 0; k < m->q.pos; ++k) ran[m->q.ord[k]] = 1;
+    for (uint32_t k = 0; k < nstim; ++k) {
+        if (!ran[k]) out_hashes[k] = 0;
+        else if (!m->q.budgets[k]) out_hashes[k] = 0xcbf29ce484222325ull;
+    }
+    return RF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+constexpr char kQueueMagic[8] = {'R', 'F', 'Q', 'U', 'E', 'U', 'E', 1};
+
+struct queue_header {
+    char magic[8];
+    uint32_t version, header_bytes;
+    uint64_t design_hash;
+    uint64_t seed;
+    uint32_t nstim, pos;
+    uint64_t total_bytes;
+    uint64_t reserved[3];
+};
+static_assert(sizeof(queue_header) == 72, "queue checkpoint header layout");
+
+size_t queue_blob_bytes(uint32_t nstim) { return sizeof(queue_header) + (size_t)nstim * 16; }
+
+}  // namespace
+
+extern "C" {
+
+int rf_queue_save_bytes(rf_model* m, size_t* bytes) {
+    if (!m || !bytes) return fail(RF_E_ARG, "null argument");
+    if (!m->q.open) return fail(RF_E_STATE, "no open queue");
+    *bytes = queue_blob_bytes(m->q.nstim);
+    return RF_OK;
+}
+
+int rf_queue_save(rf_model* m, void* dst, size_t bytes) {
+    if (!m || !dst) return fail(RF_E_ARG, "null argument");
+    if (!m->q.open) return fail(RF_E_STATE, "no open queue");
+    const uint32_t n = m->q.nstim;
+    if (bytes < queue_blob_bytes(n))
+        return fail(RF_E_ARG, "queue checkpoint needs %zu bytes, buffer has %zu", queue_blob_bytes(n), bytes);
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
+    queue_header h{};
+    memcpy(h.magic, kQueueMagic, 8);
+    h.version = 1;
+    h.header_bytes = sizeof h;
+    h.design_hash = m->d->hash;
+    h.seed = m->q.seed;
+    h.nstim = n;
+    h.pos = m->q.pos;
+    h.total_bytes = queue_blob_bytes(n);
+    uint8_t* o = (uint8_t*)dst;
+    memcpy(o, &h, sizeof h);
+    o += sizeof h;
+    if (n) {
+        memcpy(o, m->q.budgets.data(), (size_t)n * 4);
+        memcpy(o + (size_t)n * 4, m->q.ord.data(), (size_t)n * 4);
+        RF_HIP(hipMemcpy(o + (size_t)n * 8, m->out_hash, (size_t)n * 8, hipMemcpyDeviceToHost));
+    }
+    return RF_OK;
+}
+
+int rf_queue_load(rf_model* m, const void* src, size_t bytes) {
+    if (!m || !src) return fail(RF_E_ARG, "null argument");
+    queue_header h{};
+    if (bytes < sizeof h) return fail(RF_E_ARG, "queue checkpoint shorter than its header");
+    memcpy(&h, src, sizeof h);
+    if (memcmp(h.magic, kQueueMagic, 8) || h.version != 1 || h.header_bytes != sizeof h)
+        return fail(RF_E_ARG, "not a queue checkpoint of this runtime (magic or version)");
+    if (h.design_hash != m->d->hash)
+        return fail(RF_E_DESIGN, "queue checkpoint of another design (hash %llx, model %llx)",
+                    (unsigned long long)h.design_hash, (unsigned long long)m->d->hash);
+    if (h.total_bytes != queue_blob_bytes(h.nstim) || bytes < h.total_bytes || h.pos > h.nstim)
+        return fail(RF_E_ARG, "queue checkpoint size or position inconsistent");
+    const uint8_t* i = (const uint8_t*)src + sizeof h;
+    const uint32_t n = h.nstim;
+    const uint32_t* budgets = (const uint32_t*)i;
+    const uint32_t* order = (const uint32_t*)(i + (size_t)n * 4);
+    int rc = rf_queue_open(m, h.seed, budgets, n, order);   // checks the order is a permutation
+    if (rc) return rc;
+    if (n) RF_HIP(hipMemcpy(m->out_hash, i + (size_t)n * 8, (size_t)n * 8, hipMemcpyHostToDevice));
+    m->q.pos = h.pos;
+    return RF_OK;
+}
+
 int rf_set_mem_images(rf_model* m, uint32_t mi, uint32_t nstim, uint32_t nelem, const void* src) {
     if (!m) return fail(RF_E_ARG, "null model");
     {
@@ -1045,31 +1238,19 @@ int rf_submit_queue_static(rf_model* m, uint64_t seed, const uint32_t* budgets, 
                      out_hashes);
 }
 
-int rf_submit_queue_bitplane(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
-                             const uint32_t* order, uint64_t* out_hashes) {
-    if (!m || !budgets || !out_hashes) return fail(RF_E_ARG, "null argument");
-    {
-        const int src = settle(m);
-        if (src) return src;
-    }
-    if (!m->bp_prog) return fail(RF_E_STATE, "design %s has no bit-plane program", m->d->name);
-    if (m->base.images)
-        return fail(RF_E_STATE, "memory images are loaded by the per-lane batched form only");
-    if (!nstim) return RF_OK;
-    int rc = queue_stream_check(m, budgets, nstim);
-    if (rc) return rc;
-    std::vector<uint32_t> ord;
-    rc = queue_setup(m, budgets, nstim, order, ord);
-    if (rc) return rc;
-    RF_HIP(hipMemsetAsync(m->flags, 0, RF_FLAG_WORDS * 4, m->stream));
-    RF_HIP(hipMemcpyAsync(m->budget, budgets, (size_t)nstim * 4, hipMemcpyHostToDevice, m->stream));
-    RF_HIP(hipMemsetAsync(m->out_hash, 0, (size_t)nstim * 8, m->stream));
+// The bit-plane form of a batched workload over queue positions
+// ord[0, n): rounds of n_lanes stimuli, every round from the reset state;
+// each stimulus's hash goes to the device out_hash[stimulus id] (nstim: the
+// whole queue's size, for the kernel's budget lookups).
+static int bitplane_rounds(rf_model* m, uint64_t seed, const uint32_t* budgets, const uint32_t* ord,
+                           uint32_t n, uint32_t nstim) {
+    int rc = RF_OK;
     const uint32_t nl = m->nlanes;
     std::vector<uint64_t> h(m->npad, 0xcbf29ce484222325ull);
     std::vector<uint32_t> sid(m->npad, 0);
     std::vector<uint8_t> done(m->npad, 1);
-    for (uint32_t r0 = 0; r0 < nstim; r0 += nl) {
-        const uint32_t cnt = std::min(nl, nstim - r0);
+    for (uint32_t r0 = 0; r0 < n; r0 += nl) {
+        const uint32_t cnt = std::min(nl, n - r0);
         uint32_t ncyc = 0;
         for (uint32_t k = 0; k < nl; ++k) {
             done[k] = k < cnt ? 0 : 1;
@@ -1107,14 +1288,38 @@ int rf_submit_queue_bitplane(rf_model* m, uint64_t seed, const uint32_t* budgets
         m->stats.bitplane_launches++;
         m->stats.bitplane_ms += ms;
     }
+    std::vector<uint8_t> undone(m->npad, 0);
+    RF_HIP(hipMemcpyAsync(m->done, undone.data(), m->npad, hipMemcpyHostToDevice, m->stream));
+    RF_HIP(hipStreamSynchronize(m->stream));
+    return check_flags(m);
+}
+
+int rf_submit_queue_bitplane(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32_t nstim,
+                             const uint32_t* order, uint64_t* out_hashes) {
+    if (!m || !budgets || !out_hashes) return fail(RF_E_ARG, "null argument");
+    {
+        const int src = settle(m);
+        if (src) return src;
+    }
+    if (!m->bp_prog) return fail(RF_E_STATE, "design %s has no bit-plane program", m->d->name);
+    if (m->base.images)
+        return fail(RF_E_STATE, "memory images are loaded by the per-lane batched form only");
+    if (!nstim) return RF_OK;
+    int rc = queue_stream_check(m, budgets, nstim);
+    if (rc) return rc;
+    std::vector<uint32_t> ord;
+    rc = queue_setup(m, budgets, nstim, order, ord);
+    if (rc) return rc;
+    RF_HIP(hipMemsetAsync(m->flags, 0, RF_FLAG_WORDS * 4, m->stream));
+    RF_HIP(hipMemcpyAsync(m->budget, budgets, (size_t)nstim * 4, hipMemcpyHostToDevice, m->stream));
+    RF_HIP(hipMemsetAsync(m->out_hash, 0, (size_t)nstim * 8, m->stream));
+    rc = bitplane_rounds(m, seed, budgets, ord.data(), nstim, nstim);
+    if (rc) return rc;
     // stimuli of budget 0 keep the FNV offset basis
     std::vector<uint64_t> oh(nstim);
-    RF_HIP(hipMemcpyAsync(oh.data(), m->out_hash, (size_t)nstim * 8, hipMemcpyDeviceToHost, m->stream));
-    std::fill(done.begin(), done.end(), 0);
-    RF_HIP(hipMemcpyAsync(m->done, done.data(), m->npad, hipMemcpyHostToDevice, m->stream));
-    RF_HIP(hipStreamSynchronize(m->stream));
+    RF_HIP(hipMemcpy(oh.data(), m->out_hash, (size_t)nstim * 8, hipMemcpyDeviceToHost));
     for (uint32_t k = 0; k < nstim; ++k) out_hashes[k] = budgets[k] ? oh[k] : 0xcbf29ce484222325ull;
-    return check_flags(m);
+    return RF_OK;
 }
 
 // ---------------------------------------------------------------- checkpoints

@@ -81,6 +81,12 @@ def runtime() -> C.CDLL:
         "rf_submit_queue_static": (C.c_int, [P, u64, P, u32, P, P]),
         "rf_set_mem_images": (C.c_int, [P, u32, u32, u32, P]),
         "rf_stream_words": (C.c_int, [P, C.POINTER(u32), C.POINTER(u32)]),
+        "rf_queue_open": (C.c_int, [P, u64, P, u32, P]),
+        "rf_queue_step": (C.c_int, [P, u32, u32, C.POINTER(u32)]),
+        "rf_queue_hashes": (C.c_int, [P, P, u32]),
+        "rf_queue_save_bytes": (C.c_int, [P, C.POINTER(C.c_size_t)]),
+        "rf_queue_save": (C.c_int, [P, P, C.c_size_t]),
+        "rf_queue_load": (C.c_int, [P, P, C.c_size_t]),
         "rf_set_input_stream": (C.c_int, [P, P, u64, u32, u32, u32]),
         "rf_set_output_capture": (C.c_int, [P, P, u64, u32, u32, u32]),
         "rf_lane_state": (C.c_int, [P, u32, P, C.POINTER(C.c_size_t)]),
@@ -255,6 +261,37 @@ class Model:
             raise ValueError(f"no batched workload in mode {mode}")
         _check(fn(self.h, seed & ((1 << 64) - 1), b.ctypes.data, len(b), optr, out.ctypes.data))
         return out
+
+    # ------------------------------------------------------- resumable queue
+    def queue_open(self, seed: int, budgets: Sequence[int], order=None):
+        b = np.ascontiguousarray(budgets, dtype=np.uint32)
+        o = None if order is None else np.ascontiguousarray(order, dtype=np.uint32)
+        _check(self.rt.rf_queue_open(self.h, seed & ((1 << 64) - 1), b.ctypes.data, len(b),
+                                     None if o is None else o.ctypes.data))
+        self._qn = len(b)
+
+    def queue_step(self, max_stimuli: int, mode: int = MODE_FUSED) -> int:
+        """Run the next ``max_stimuli`` queue positions; returns the position."""
+        pos = C.c_uint32(0)
+        _check(self.rt.rf_queue_step(self.h, max_stimuli, mode, C.byref(pos)))
+        return pos.value
+
+    def queue_hashes(self) -> np.ndarray:
+        out = np.zeros(self._qn, dtype=np.uint64)
+        _check(self.rt.rf_queue_hashes(self.h, out.ctypes.data, self._qn))
+        return out
+
+    def queue_save(self) -> bytes:
+        n = C.c_size_t(0)
+        _check(self.rt.rf_queue_save_bytes(self.h, C.byref(n)))
+        buf = (C.c_uint8 * max(1, n.value))()
+        _check(self.rt.rf_queue_save(self.h, buf, n.value))
+        return bytes(buf)[:n.value]
+
+    def queue_load(self, blob: bytes):
+        buf = (C.c_uint8 * len(blob)).from_buffer_copy(blob)
+        _check(self.rt.rf_queue_load(self.h, buf, len(blob)))
+        self._qn = int(np.frombuffer(blob[32:36], np.uint32)[0])
 
     def lane_state(self, lane: int) -> bytes:
         n = C.c_size_t(0)
