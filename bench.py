@@ -130,7 +130,8 @@ def main():
                     help="stimuli in the netlist CPU baseline sample (x the full cycles)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--main-only", action="store_true",
-                    help="other workloads: run only the reported form (for counter profiles)")
+                    help="run only the reported form: the headline's bit-plane steps, or another "
+                         "workload's main leg (for counter profiles)")
     ap.add_argument("--budget-s", type=float, default=300.0,
                     help="optional legs (other forms, learned partition, CPU baseline) are "
                          "skipped once this many seconds have passed")
@@ -175,6 +176,10 @@ def bench_netlist(args, torch):
             return False
         return True
 
+    if args.main_only:
+        # only the bit-plane steps (and their oracle sample)
+        args.fused_cycles = args.graph_cycles = args.learned_cycles = args.stream_cycles = 0
+        args.no_cpu = True
     seed = registry.SEED_CFG[2]
     cd = registry.get(args.design)
     man = cd.manifest

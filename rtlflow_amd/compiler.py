@@ -669,7 +669,39 @@ def _hash_code(comp: Compiled, gen: Gen, h: str, sid: Optional[str] = None, cyc:
         gen.emit("}")
 
 
-def _stim_code(comp: Compiled, gen: Gen, sid: str, cyc: str):
+# cycle loops fetch the next cycle's input words one cycle ahead, into
+# registers, when the design has at most this many
+PREFETCH_WORDS = 8
+
+
+def _input_words(comp: Compiled) -> List[Tuple[int, int]]:
+    """(generator key, stream word) of every input word, in stream order."""
+    out = []
+    for port in comp.port_table():
+        if port["dir"] == 0:
+            for j in range((port["width"] + 63) // 64):
+                out.append(((port["stim_index"] << 3) | j, len(out)))
+    return out
+
+
+def _stim_prefetch(comp: Compiled, gen: Gen, sid: str, cyc: str, live: str, declare: bool) -> bool:
+    """Fetch the input words of cycle ``cyc`` into registers rfin<n> (the
+    double buffer of a cycle loop: issued a cycle before their use, so the
+    loads of a stream overlap the current cycle's logic). ``live``: the lane
+    runs that cycle. Returns False (and emits nothing) for designs with more
+    than PREFETCH_WORDS input words."""
+    words = _input_words(comp)
+    if not words or len(words) > PREFETCH_WORDS:
+        return False
+    for key, n in words:
+        lhs = f"uint64_t rfin{n}" if declare else f"rfin{n}"
+        gen.emit(f"{lhs} = rf_input(a, {sid}, {cyc}, {key}u, {n}u, {live});")
+    return True
+
+
+def _stim_code(comp: Compiled, gen: Gen, sid: str, cyc: str, prefetched: bool = False):
+    """Apply one cycle's inputs: from rf_input, or from the registers
+    _stim_prefetch filled when ``prefetched``."""
     n = 0
     for port in comp.port_table():
         if port["dir"] != 0:
@@ -679,7 +711,8 @@ def _stim_code(comp: Compiled, gen: Gen, sid: str, cyc: str):
         p = port["stim_index"]
         if is_wide(v.width):
             for j in range((v.width + 63) // 64):
-                gen.emit(f"{{ const uint64_t r = rf_input(a, {sid}, {cyc}, {(p << 3) | j}u, {n}u);")
+                src = f"rfin{n}" if prefetched else f"rf_input(a, {sid}, {cyc}, {(p << 3) | j}u, {n}u)"
+                gen.emit(f"{{ const uint64_t r = {src};")
                 n += 1
                 gen.emit(f"  {gen.name(v)}[{2 * j}] = (uint32_t)r;")
                 if 2 * j + 1 < v.words:
@@ -688,8 +721,8 @@ def _stim_code(comp: Compiled, gen: Gen, sid: str, cyc: str):
                     gen.emit("}")
             gen.emit(f"rfw_clean({gen.name(v)}, {v.width});")
         else:
-            gen.emit(gen.set(v, f"({cty(v.width)})(rf_input(a, {sid}, {cyc}, {p << 3}u, {n}u) & "
-                                f"{lit((1 << v.width) - 1, 64)})"))
+            src = f"rfin{n}" if prefetched else f"rf_input(a, {sid}, {cyc}, {p << 3}u, {n}u)"
+            gen.emit(gen.set(v, f"({cty(v.width)})({src} & {lit((1 << v.width) - 1, 64)})"))
             n += 1
         gen.avail.add(v)
         gen.dirty.add(v)
@@ -1163,9 +1196,12 @@ def emit_fused(comp: Compiled) -> str:
     gen.emit("uint64_t h = a.hash[lane];")
     gen.emit("uint32_t cyc = a.cyc[lane];")
     gen.emit("const uint64_t sid = (uint64_t)a.stim[lane];")
+    pf = _stim_prefetch(comp, gen, "sid", "(uint64_t)cyc", "a.ncycles != 0u", True)
     gen.emit("for (uint32_t n = 0u; n < a.ncycles; ++n, ++cyc) {")
     gen.ind += 1
-    _stim_code(comp, gen, "sid", "(uint64_t)cyc")
+    _stim_code(comp, gen, "sid", "(uint64_t)cyc", pf)
+    if pf:
+        _stim_prefetch(comp, gen, "sid", "(uint64_t)cyc + 1u", "n + 1u < a.ncycles", False)
     gen.emit("#pragma unroll 1")
     gen.emit("for (uint32_t ph = 0u; ph < 2u; ++ph) {")
     gen.ind += 1

@@ -16,7 +16,7 @@ rule, which bounds the tail by the longest single stimulus).
 from __future__ import annotations
 
 from .compiler import (_eval_loop_body_split, _fused_locals, clk0_carried, clk0_cone, _hash_code, _initial_body, _lds_copy_lines,
-                       _lds_decl_lines, _prologue, _stim_code, atomic, kernel_header, lds_memories,
+                       _lds_decl_lines, _prologue, _stim_code, _stim_prefetch, atomic, kernel_header, lds_memories,
                        place_lds_state, plan_packing)
 from .emit_hip import Gen, is_wide
 from .ir import CLASS_BYTES
@@ -101,9 +101,12 @@ def emit_persistent(comp) -> str:
         gen.emit(ln)
     _initial_body(comp, gen)
     gen.emit("uint64_t h = RF_FNV_BASIS;")
+    pf = _stim_prefetch(comp, gen, "(uint64_t)sid", "0ull", "budget != 0u", True)
     gen.emit("for (uint32_t cyc = 0u; cyc < budget; ++cyc) {")
     gen.ind += 1
-    _stim_code(comp, gen, "(uint64_t)sid", "(uint64_t)cyc")
+    _stim_code(comp, gen, "(uint64_t)sid", "(uint64_t)cyc", pf)
+    if pf:
+        _stim_prefetch(comp, gen, "(uint64_t)sid", "(uint64_t)cyc + 1u", "cyc + 1u < budget", False)
     gen.emit("#pragma unroll 1")
     gen.emit("for (uint32_t ph = 0u; ph < 2u; ++ph) {")
     gen.ind += 1

@@ -135,11 +135,13 @@ def test_generator_words_reproduce_generator_at_scale(gpu, mode):
     nout = man["streams"]["out_words"]
     cap_gen = torch.zeros((nc, nout, n), dtype=torch.int64, device="cuda")
     cap_str = torch.zeros_like(cap_gen)
+    # two fresh models: a begin restarts the stimuli, not the design state
     with cd.model(n) as m:
         m.begin(SEED)
         m.set_output_capture(cap_gen)
         m.run_cycles(nc, mode)
         h_gen = m.hashes()
+    with cd.model(n) as m:
         m.begin(0)
         m.set_input_stream(_dev(_generator_words(man, SEED, n, nc)))
         m.set_output_capture(cap_str)
@@ -160,13 +162,12 @@ def test_partial_capture_window_bitplane(gpu):
     n, nc = 256, 20
     full = torch.zeros((nc, nout, n), dtype=torch.int64, device="cuda")
     part = torch.zeros((10, nout, 64), dtype=torch.int64, device="cuda")
-    with cd.model(n) as m:
-        m.begin(SEED)
-        m.set_output_capture(full)
-        m.run_cycles(nc, MODE_BITPLANE)
-        m.begin(SEED)
-        m.set_output_capture(part, stim0=70, cycle0=5)
-        m.run_cycles(nc, MODE_BITPLANE)
+    for buf, s0, c0 in ((full, 0, 0), (part, 70, 5)):
+        with cd.model(n) as m:
+            m.begin(SEED)
+            m.set_output_capture(buf, stim0=s0, cycle0=c0)
+            m.run_cycles(nc, MODE_BITPLANE)
+            m.sync()
     assert torch.equal(part, full[5:15, :, 70:134])
 
 
@@ -189,6 +190,8 @@ def test_stream_errors(gpu):
         m.begin(1)                           # clears the report
         m.run_cycles(4, MODE_BITPLANE)
         m.hashes()
+    with cd.model(64) as m:
+        m.set_input_stream(x)
         m.set_input_stream(None)             # back to the generator
         m.begin(SEED)
         m.run_cycles(6, MODE_FUSED)
