@@ -210,7 +210,9 @@ register("rv32i_learned", _rv32i(), learned_threads="rv32i", bitplane=False)
 register("net2_lmerge", _netlist(seed=SEED_CFG[2], name="net2"), learned_threads="net2",
          learned_merge=True, bitplane=False)
 register("systolic_lmerge", _systolic(), learned_threads="systolic", learned_merge=True, bitplane=False)
-register("net5s_lmerge", ENTRIES["net5s"].make, learned_threads="net5s", learned_merge=True, bitplane=False)
+# net5s with learned merges is not registered: its graph form hit a GPU
+# memory fault whose cause is not found (the host build of the same generated
+# kernels is bit-exact and clean under AddressSanitizer)
 register("rv32i_lmerge", _rv32i(), learned_threads="rv32i", learned_merge=True, bitplane=False)
 
 # the autotuner's choice of compile-time options, where it measured one
