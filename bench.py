@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--cpu-sample-netlist", type=int, default=1024,
                     help="stimuli in the netlist CPU baseline sample (x the full cycles)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--budget-cap", type=int, default=0,
+                    help="net5: cap every stimulus's budget (0: none; for short counter profiles)")
     ap.add_argument("--main-only", action="store_true",
                     help="run only the reported form: the headline's bit-plane steps, or another "
                          "workload's main leg (for counter profiles)")
@@ -299,19 +301,7 @@ def bench_netlist(args, torch):
     # measured limits of the same launch shape from the committed SQ counters:
     # VALU issue (wave instructions over one per CU per clock), LDS bank
     # conflict cycles per CU-cycle, HBM traffic over peak bandwidth
-    limits = None
-    q = profile_json(f"{args.design}_bitplane_sq.json")
-    if q and all(q.get(k) == v for k, v in shape.items()):
-        ks = q["kernel_ns"] * 1e-9
-        cu_cycles = CUS * CLOCK_HZ * ks
-        limits = {"valu_issue": q["SQ_INSTS_VALU"] / cu_cycles,
-                  "lds_instr_issue": q["SQ_INSTS_LDS"] / cu_cycles,
-                  "lds_bank_conflict": q["SQ_LDS_BANK_CONFLICT"] / cu_cycles,
-                  "hbm_traffic": (traffic / ks / (HBM_PEAK_GBS * 1e9)) if traffic else None,
-                  "source": f"profiles/{args.design}_bitplane_sq.json",
-                  "kernel_ns": q["kernel_ns"]}
-        cand = {k: v for k, v in limits.items() if isinstance(v, float)}
-        limits["highest"] = max(cand, key=cand.get)
+    limits = sq_limits(f"{args.design}_bitplane_sq.json", shape, traffic)
     gates_m = man["bitplane"]["gates"][1]
 
     cpu = None
@@ -402,22 +392,24 @@ def bench_stream(args, torch, cd, man, lanes, nt, seed):
         m.checkpoint_to(ck.data_ptr(), nb)
         m.run_cycles(sc, MODE_BITPLANE)
         ref = m.hashes()
-        for tag, use_cap in (("stream", False), ("stream_capture", True)):
-            m.set_input_stream(xin)
+        for tag, use_in, use_cap in (("generator", False, False), ("stream", True, False),
+                                     ("stream_capture", True, True)):
+            m.set_input_stream(xin if use_in else None)
             m.set_output_capture(cap if use_cap else None)
             m.sync()
             m.reset_stats()
             t0 = time.perf_counter()
             for _ in range(reps):
                 m.restore_from(ck.data_ptr(), nb)
-                m.begin(0)
+                m.begin(0 if use_in else seed)
                 m.run_cycles(sc, MODE_BITPLANE)
             m.sync()
             dt = time.perf_counter() - t0
             st = m.stats()
             same = bool((m.hashes() == ref).all())
             per_ms = st["bitplane_ms"] / max(1, st["bitplane_launches"])
-            b = 2 * ((man["bitplane"]["state_bits"] + 7) // 8) + 8 * nin + (8 * nout if use_cap else 0)
+            b = 2 * ((man["bitplane"]["state_bits"] + 7) // 8) + (8 * nin if use_in else 0) + \
+                (8 * nout if use_cap else 0)
             out[tag] = {"value": reps * lanes * sc / dt, "kernel_value": lanes * sc / (per_ms * 1e-3),
                         "avg_launch_ms": per_ms, "alg_bytes_per_stimulus_cycle": b,
                         "hbm_frac": b * lanes * sc / (per_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
@@ -427,6 +419,8 @@ def bench_stream(args, torch, cd, man, lanes, nt, seed):
                 f"hashes equal the generator run: {same}")
         m.set_input_stream(None)
         m.set_output_capture(None)
+    for tag in ("stream", "stream_capture"):
+        out[tag]["kernel_vs_generator"] = out[tag]["kernel_value"] / out["generator"]["kernel_value"]
     del xin, cap
     torch.cuda.empty_cache()
     return out
@@ -438,15 +432,42 @@ def bench_stream(args, torch, cd, man, lanes, nt, seed):
 T_MEM = {"alu": 12, "systolic": 0, "rv32i": 20}
 
 
-def roofline_line(b_alg, stim_cycles_per_launch, per_launch_ms, kernel, traffic_json=None, shape=None):
+def sq_limits(sq_json, shape, traffic):
+    """Measured limits of a launch shape from its committed SQ counters (one
+    dispatch): VALU and LDS instruction issue (wave instructions over one per
+    CU per clock), LDS bank-conflict cycles per CU-cycle, HBM traffic over
+    peak bandwidth; None when no profile of that shape is committed."""
+    q = profile_json(sq_json)
+    if not (q and all(q.get(k) == v for k, v in shape.items())):
+        return None
+    ks = q["kernel_ns"] * 1e-9
+    cu_cycles = CUS * CLOCK_HZ * ks
+    limits = {"valu_issue": q["SQ_INSTS_VALU"] / cu_cycles,
+              "lds_instr_issue": q["SQ_INSTS_LDS"] / cu_cycles,
+              "lds_bank_conflict": q["SQ_LDS_BANK_CONFLICT"] / cu_cycles,
+              "hbm_traffic": (traffic / ks / (HBM_PEAK_GBS * 1e9)) if traffic else None,
+              "source": f"profiles/{sq_json}", "kernel_ns": q["kernel_ns"]}
+    cand = {k: limits[k] for k in ("valu_issue", "lds_instr_issue", "lds_bank_conflict", "hbm_traffic")
+            if isinstance(limits[k], float)}
+    limits["highest"] = max(cand, key=cand.get)
+    return limits
+
+
+def roofline_line(b_alg, stim_cycles_per_launch, per_launch_ms, kernel, traffic_json=None, shape=None,
+                  sq_json=None):
     achieved = b_alg * stim_cycles_per_launch / (per_launch_ms * 1e-3) / 1e9
     traffic = None
     t = profile_json(traffic_json) if traffic_json else None
     if t and shape and all(t.get(k) == v for k, v in shape.items()):
         traffic = t["read_bytes"] + t["write_bytes"]
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
-            "alg_bytes_per_stimulus_cycle": b_alg, "avg_launch_ms": per_launch_ms}
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
+           "alg_bytes_per_stimulus_cycle": b_alg, "avg_launch_ms": per_launch_ms}
+    if shape:
+        out["shape"] = shape
+    if sq_json and shape:
+        out["limits"] = sq_limits(sq_json, shape, traffic)
+    return out
 
 
 def bench_other(args):
@@ -501,9 +522,12 @@ def bench_other(args):
             ok = bool((got == ref).all())
             if mode == MODE_FUSED:
                 per = st["fused_ms"] / max(1, st["fused_launches"])
+                # the committed counters are the reported (last) form's
                 rl = roofline_line(b_alg, lanes * min(chunk, cycles), per, "rf_k_fused",
-                                   f"{wl}_fused_traffic.json",
-                                   {"lanes": lanes, "cycles_per_launch": min(chunk, cycles)})
+                                   f"{wl}_{form}_traffic.json",
+                                   {"lanes": lanes, "cycles_per_launch": min(chunk, cycles),
+                                    "block": c.manifest["block"]},
+                                   f"{wl}_{form}_sq.json")
             else:
                 per = st["graph_ms"] / max(1, st["graph_launches"])
                 rl = roofline_line(b_alg, lanes, per, "cycle graph")
@@ -552,7 +576,8 @@ def bench_other(args):
             outs[form] = out
             ok = bool((out[:nchk] == ref).all()) and bool((out == outs.get("batched", out)).all())
             rl = roofline_line(b_alg, total, st["fused_ms"] / max(1, st["fused_launches"]),
-                               "rf_k_persistent")
+                               "rf_k_persistent", f"{wl}_persistent_traffic.json",
+                               {"lanes": ln, "stimuli": len(budgets)}, f"{wl}_persistent_sq.json")
             forms[form] = {"seconds": dt, "value": total / dt, "bit_exact_sample": ok,
                            "checked_stimuli": nchk, "lanes": ln, "roofline": rl}
             log(f"[bench] {form} {dt:.3f}s {total / dt:.4g} stim-cycles/s oracle_match={ok}")
@@ -673,7 +698,9 @@ def bench_net5s(args):
         # stimulus's own budget) over the launches
         b_alg = 2 * ((cd.manifest["bitplane"]["state_bits"] + 7) // 8) + 8 * 16
         rl = roofline_line(b_alg, total / max(1, st["bitplane_launches"]),
-                           st["bitplane_ms"] / max(1, st["bitplane_launches"]), f"rf_bp_kernel<{nt}>")
+                           st["bitplane_ms"] / max(1, st["bitplane_launches"]), f"rf_bp_kernel<{nt}>",
+                           "net5s_bitplane_traffic.json", {"lanes": lanes, "bitplane_threads": nt},
+                           "net5s_bitplane_sq.json")
         forms["bitplane"] = {"seconds": dt, "value": total / dt, "bit_exact_sample": ok,
                              "checked_stimuli": nchk, "lanes": lanes, "threads": nt,
                              "stimulus_cycles": total, "roofline": rl}
@@ -768,6 +795,8 @@ def bench_net5(args):
     seed = registry.SEED_CFG[5]
     nstim = 4096
     budgets = rv32i.budgets(seed, nstim, median=2000, lo=200, hi=20000)
+    if args.budget_cap:
+        budgets = np.minimum(budgets, args.budget_cap)
     total = int(budgets.astype(np.int64).sum())
     tb = registry.gatenet_table("net5")
     meta = registry.gatenet_meta("net5")

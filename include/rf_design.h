@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define RF_DESIGN_ABI 6
+#define RF_DESIGN_ABI 7
 
 /* storage classes (include/rf_heavy.h:11-17: CData/SData/IData/QData) */
 enum { RF_CLS_C = 0, RF_CLS_S = 1, RF_CLS_I = 2, RF_CLS_Q = 3 };
@@ -137,6 +137,11 @@ typedef struct rf_design {
                                   captured passes finish the settle loop
                                   (NULL when nchange == 0); rf_args.ncycles =
                                   passes already run */
+    /* k_fused / k_persistent with the input-stream and output-capture code:
+     * the runtime launches these when rf_args.in or rf_args.out is set (the
+     * plain kernels keep that code out of their cycle loops) */
+    const void* k_fused_io;
+    const void* k_persistent_io;
     uint32_t nchange;          /* change-detected variables (0: one pass settles) */
     uint32_t clk_port;         /* index into ports, or ~0u */
     /* algorithmic bytes per stimulus-cycle (layout manifest) */

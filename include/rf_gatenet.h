@@ -20,7 +20,11 @@
  *   RF_GN_BITPLANE  64 stimuli per 64-bit plane, planes of all 64-lane
  *                   groups of one node contiguous (one coalesced access per
  *                   wavefront for 64 groups), one kernel per level, the
- *                   clk = 1 eval limited to the outputs' cone.
+ *                   clk = 1 eval limited to the outputs' cone. Stimuli are
+ *                   dealt to lanes longest budget first, and each cycle runs
+ *                   over the first power-of-two count of groups that covers
+ *                   every group with a lane still running (one graph per
+ *                   count), so finished groups cost nothing.
  *
  * No counterpart in the reference, which compiles every design through its
  * emitter (src/V3EmitC.cpp); the protocol and hash are the benchmark's.

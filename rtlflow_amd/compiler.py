@@ -699,9 +699,10 @@ def _stim_prefetch(comp: Compiled, gen: Gen, sid: str, cyc: str, live: str, decl
     return True
 
 
-def _stim_code(comp: Compiled, gen: Gen, sid: str, cyc: str, prefetched: bool = False):
+def _stim_code(comp: Compiled, gen: Gen, sid: str, cyc: str, prefetched: bool = False, io: bool = True):
     """Apply one cycle's inputs: from rf_input, or from the registers
-    _stim_prefetch filled when ``prefetched``."""
+    _stim_prefetch filled when ``prefetched``; without ``io`` from the
+    generator only (no stream code in the kernel)."""
     n = 0
     for port in comp.port_table():
         if port["dir"] != 0:
@@ -711,7 +712,8 @@ def _stim_code(comp: Compiled, gen: Gen, sid: str, cyc: str, prefetched: bool = 
         p = port["stim_index"]
         if is_wide(v.width):
             for j in range((v.width + 63) // 64):
-                src = f"rfin{n}" if prefetched else f"rf_input(a, {sid}, {cyc}, {(p << 3) | j}u, {n}u)"
+                src = f"rfin{n}" if prefetched else (f"rf_input(a, {sid}, {cyc}, {(p << 3) | j}u, {n}u)" if io
+                                                     else f"rf_stim(a.seed, {sid}, {cyc}, {(p << 3) | j}u)")
                 gen.emit(f"{{ const uint64_t r = {src};")
                 n += 1
                 gen.emit(f"  {gen.name(v)}[{2 * j}] = (uint32_t)r;")
@@ -721,7 +723,8 @@ def _stim_code(comp: Compiled, gen: Gen, sid: str, cyc: str, prefetched: bool = 
                     gen.emit("}")
             gen.emit(f"rfw_clean({gen.name(v)}, {v.width});")
         else:
-            src = f"rfin{n}" if prefetched else f"rf_input(a, {sid}, {cyc}, {p << 3}u, {n}u)"
+            src = f"rfin{n}" if prefetched else (f"rf_input(a, {sid}, {cyc}, {p << 3}u, {n}u)" if io
+                                                 else f"rf_stim(a.seed, {sid}, {cyc}, {p << 3}u)")
             gen.emit(gen.set(v, f"({cty(v.width)})({src} & {lit((1 << v.width) - 1, 64)})"))
             n += 1
         gen.avail.add(v)
@@ -1176,8 +1179,11 @@ def _lds_copy_lines(comp: Compiled, lds: Dict[Var, str], to_lds: bool) -> List[s
     return out
 
 
-def emit_fused(comp: Compiled) -> str:
-    """Protocol kernel: ``a.ncycles`` cycles per lane with all logic inlined."""
+def emit_fused(comp: Compiled, io: bool = False) -> str:
+    """Protocol kernel: ``a.ncycles`` cycles per lane with all logic inlined.
+    ``io``: the variant that reads an input stream (inputs fetched a cycle
+    ahead) and fills an output capture (rf_k_fused_io); the plain variant has
+    neither in its cycle loop."""
     d = comp.design
     gen = Gen(comp)
     gen.lds = lds_memories(comp)
@@ -1196,10 +1202,10 @@ def emit_fused(comp: Compiled) -> str:
     gen.emit("uint64_t h = a.hash[lane];")
     gen.emit("uint32_t cyc = a.cyc[lane];")
     gen.emit("const uint64_t sid = (uint64_t)a.stim[lane];")
-    pf = _stim_prefetch(comp, gen, "sid", "(uint64_t)cyc", "a.ncycles != 0u", True)
+    pf = io and _stim_prefetch(comp, gen, "sid", "(uint64_t)cyc", "a.ncycles != 0u", True)
     gen.emit("for (uint32_t n = 0u; n < a.ncycles; ++n, ++cyc) {")
     gen.ind += 1
-    _stim_code(comp, gen, "sid", "(uint64_t)cyc", pf)
+    _stim_code(comp, gen, "sid", "(uint64_t)cyc", pf, io)
     if pf:
         _stim_prefetch(comp, gen, "sid", "(uint64_t)cyc + 1u", "n + 1u < a.ncycles", False)
     gen.emit("#pragma unroll 1")
@@ -1211,7 +1217,10 @@ def emit_fused(comp: Compiled) -> str:
     _eval_loop_body_split(comp, gen, "n == 0u", cone)
     gen.ind -= 1
     gen.emit("}")
-    _hash_code(comp, gen, "h", "sid", "(uint64_t)cyc")
+    if io:
+        _hash_code(comp, gen, "h", "sid", "(uint64_t)cyc")
+    else:
+        _hash_code(comp, gen, "h")
     gen.ind -= 1
     gen.emit("}")
     gen.emit("a.hash[lane] = h;")
@@ -1220,7 +1229,7 @@ def emit_fused(comp: Compiled) -> str:
     for v in sorted(gen.dirty | (gen.local & comp.stored), key=lambda x: x.id):
         if v in comp.stored:
             gen.store(v)
-    lines = [kernel_header(comp, f"rf_k_fused")]
+    lines = [kernel_header(comp, "rf_k_fused_io" if io else "rf_k_fused")]
     lines += _lds_decl_lines(comp, gen.lds) + lds_state
     _prologue(comp, gen, lines, check_done=True)
     lines += _lds_copy_lines(comp, gen.lds, to_lds=True)
@@ -1302,11 +1311,12 @@ def _emit_source(comp: Compiled) -> str:
     else:
         parts.append(emit_init(comp))
         parts.append("")
-        parts.append(emit_fused(comp))
-        parts.append("")
         from .persistent import emit_persistent
-        parts.append(emit_persistent(comp))
-        parts.append("")
+        for io in (False, True):
+            parts.append(emit_fused(comp, io))
+            parts.append("")
+            parts.append(emit_persistent(comp, io))
+            parts.append("")
     tail = emit_settle(comp)
     if tail:
         parts.append(tail)
@@ -1374,6 +1384,12 @@ def emit_descriptor(comp: Compiled, h: int) -> str:
     out.append("    (const void*)&rf_k_init, (const void*)&rf_k_eval_end, (const void*)&rf_k_stim,")
     out.append("    (const void*)&rf_k_fused, (const void*)&rf_k_persistent, (const void*)&rf_k_reset,")
     out.append("    (const void*)&rf_k_settle," if comp.sch.change_vars else "    0,")
+    # the calls form's kernels read streams through the shared stimulus and
+    # hash bodies: one kernel serves both entries
+    if comp.calls:
+        out.append("    (const void*)&rf_k_fused, (const void*)&rf_k_persistent,")
+    else:
+        out.append("    (const void*)&rf_k_fused_io, (const void*)&rf_k_persistent_io,")
     out.append(f"    {len(comp.sch.change_vars)}u, {clk_idx}u,")
     out.append(f"    {man['bytes_seq']}ull, 0ull, {man['bytes_in']}ull, {man['bytes_out']}ull,")
     out.append(f"    {man['ops_per_cycle']}ull,")

@@ -190,12 +190,18 @@ typedef __attribute__((address_space(3))) uint32_t lds_u32;
 // clk = 0 settle. Chunk k of the table is input word k (the generator's or
 // the input stream's).
 // ``live``: the lane runs that cycle (inactive lanes and lanes past their
-// budget still fill their plane bits, which nothing reads).
-template <typename TP>
+// budget still fill their plane bits, which nothing reads). IO: the launch
+// may read an input stream (otherwise the generator only, with no stream
+// code in the cycle loop).
+template <bool IO, typename TP>
 __device__ __forceinline__ void bp_inputs(uint64_t* L, TP ic, uint32_t n_in, const rf_args& a, uint64_t sid,
                                           uint32_t cyc, uint32_t t, bool live) {
     for (uint32_t k = 0; k < n_in; ++k, ic += 4) {
-        const uint64_t r = rf_input(a, sid, (uint64_t)cyc, ic[0], k, live);
+        uint64_t r;
+        if constexpr (IO)
+            r = rf_input(a, sid, (uint64_t)cyc, ic[0], k, live);
+        else
+            r = rf_stim(a.seed, sid, (uint64_t)cyc, ic[0]);
         bp_to_planes(L, ic[2], r, ic[1], t);
     }
 }
@@ -210,10 +216,12 @@ __device__ __forceinline__ void bp_copy_inputs(uint64_t* L, TP ic, uint32_t n_in
 
 // Fold this cycle's outputs into the lane's hash (wavefront 1); ``hp``: the
 // output-slot table of program M or B. With ``cap`` the outputs' 64-bit
-// words also go to the output capture as stimulus ``sid``'s cycle ``c``.
-template <typename TP>
+// words also go to the output capture as stimulus ``sid``'s cycle ``c``
+// (IO launches only).
+template <bool IO, typename TP>
 __device__ __forceinline__ uint64_t bp_hash(const uint64_t* L, TP hp, uint32_t n_hash, uint64_t h, uint32_t t,
                                             const rf_args& a, bool cap, uint64_t sid, uint64_t c) {
+    if constexpr (!IO) cap = false;
     uint32_t n = 0;
     for (uint32_t k = 0; k < n_hash; ++k) {
         const uint32_t nbytes = hp[0], width = hp[1];
@@ -377,7 +385,7 @@ __device__ __forceinline__ void bp_rounds(uint64_t* L, const uint2 (&d)[MAXR], c
 template <int NT, int MAXR>
 constexpr int bp_waves_per_eu() { return NT >= 1024 ? 4 : (MAXR > 0 && MAXR <= 24 ? 6 : 1); }
 
-template <int NT, int MAXR, bool BA>
+template <int NT, int MAXR, bool BA, bool IO>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu<NT, MAXR>())))
 rf_bp_kernel(rf_bp_params P) {
     extern __shared__ uint64_t L[];
@@ -457,7 +465,7 @@ rf_bp_kernel(rf_bp_params P) {
         }
     }
     const uint32_t* in_tab = prog + P.off_in;
-    if (w0 && a.ncycles) bp_inputs(L, in_tab, P.n_in, a, sid, cyc, t, act && lim > 0);
+    if (w0 && a.ncycles) bp_inputs<IO>(L, in_tab, P.n_in, a, sid, cyc, t, act && lim > 0);
     // per-cycle tables into LDS after the planes
     const lds_u32* tc = reinterpret_cast<const lds_u32*>((size_t)P.tab_off);
     const lds_u32* th = tc + P.n_commit[1];
@@ -502,7 +510,7 @@ rf_bp_kernel(rf_bp_params P) {
         if (P.n_mem) bp_mem_ports(L, P, 0, wave, NW, t, lane, inb, act);
         if (w0) {
             bp_copy_inputs(L, in_tab, P.n_in, t);
-            if (N > 1) bp_inputs(L, in_tab, P.n_in, a, sid, cyc + 1, t, act && lim > 1);
+            if (N > 1) bp_inputs<IO>(L, in_tab, P.n_in, a, sid, cyc + 1, t, act && lim > 1);
         }
         __syncthreads();
         // cycle n's outputs and cycle n + 1's next state in one pass
@@ -517,17 +525,18 @@ rf_bp_kernel(rf_bp_params P) {
                 bp_commit<NT>(L, prog + P.off_commit[1], P.n_commit[1], tid);
             if (P.n_mem) bp_mem_ports(L, P, 1, wave, NW, t, lane, inb, act);
             if (w1) {
-                const bool cap = a.out.data && act && n < lim;
-                const uint64_t hn = P.tab_off ? bp_hash(L, th, P.n_hash, h, t, a, cap, sid, cyc + n)
-                                              : bp_hash(L, prog + P.off_hash[0], P.n_hash, h, t, a, cap, sid, cyc + n);
+                const bool cap = IO && a.out.data && act && n < lim;
+                const uint64_t hn = P.tab_off ? bp_hash<IO>(L, th, P.n_hash, h, t, a, cap, sid, cyc + n)
+                                              : bp_hash<IO>(L, prog + P.off_hash[0], P.n_hash, h, t, a, cap, sid,
+                                                            cyc + n);
                 if (act && n < lim) h = hn;
             } else if (w0) {
                 if (P.tab_off) {
                     bp_copy_inputs(L, ti, P.n_in, t);
-                    if (n + 2 < N) bp_inputs(L, ti, P.n_in, a, sid, cyc + n + 2, t, act && n + 2 < lim);
+                    if (n + 2 < N) bp_inputs<IO>(L, ti, P.n_in, a, sid, cyc + n + 2, t, act && n + 2 < lim);
                 } else {
                     bp_copy_inputs(L, in_tab, P.n_in, t);
-                    if (n + 2 < N) bp_inputs(L, in_tab, P.n_in, a, sid, cyc + n + 2, t, act && n + 2 < lim);
+                    if (n + 2 < N) bp_inputs<IO>(L, in_tab, P.n_in, a, sid, cyc + n + 2, t, act && n + 2 < lim);
                 }
             }
             __syncthreads();
@@ -535,8 +544,8 @@ rf_bp_kernel(rf_bp_params P) {
         // the last cycle's outputs
         bp_phase<NT, BA>(L, lev[RF_BP_B], gates[RF_BP_B], P.nlev[RF_BP_B], tid);
         if (w1) {
-            const bool cap = a.out.data && act && N - 1 < lim;
-            const uint64_t hn = bp_hash(L, prog + P.off_hash[1], P.n_hash, h, t, a, cap, sid, cyc + N - 1);
+            const bool cap = IO && a.out.data && act && N - 1 < lim;
+            const uint64_t hn = bp_hash<IO>(L, prog + P.off_hash[1], P.n_hash, h, t, a, cap, sid, cyc + N - 1);
             if (act && N - 1 < lim) h = hn;
         }
         __syncthreads();
@@ -816,19 +825,28 @@ static uint32_t bp_tab_words(const rf_bp_params& p) {
     return p.n_commit[1] + p.hash_words + 4u * p.n_in;
 }
 
-template <int NT, int MAXR, bool BA>
-static hipError_t bp_launch_ba(const rf_bp_params& p, hipStream_t stream) {
+template <int NT, int MAXR, bool BA, bool IO>
+static hipError_t bp_launch_io(const rf_bp_params& p, hipStream_t stream) {
     const size_t shmem = (size_t)p.nslots * 8u + (p.tab_off ? 4u * (size_t)bp_tab_words(p) : 0u);
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rf_bp_kernel<NT, MAXR, BA>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rf_bp_kernel<NT, MAXR, BA, IO>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     const uint32_t grid = (p.a.nlanes + 63u) / 64u;
-    hipLaunchKernelGGL((rf_bp_kernel<NT, MAXR, BA>), dim3(grid), dim3(NT), shmem, stream, p);
+    hipLaunchKernelGGL((rf_bp_kernel<NT, MAXR, BA, IO>), dim3(grid), dim3(NT), shmem, stream, p);
     return hipGetLastError();
+}
+
+// The stream-capable instance only for launches with an input stream or an
+// output capture: its extra registers cost the generator-only launches
+// spills in the cycle loop.
+template <int NT, int MAXR, bool BA>
+static hipError_t bp_launch_ba(const rf_bp_params& p, hipStream_t stream) {
+    return (p.a.in.data || p.a.out.data) ? bp_launch_io<NT, MAXR, BA, true>(p, stream)
+                                         : bp_launch_io<NT, MAXR, BA, false>(p, stream);
 }
 
 template <int NT, int MAXR>

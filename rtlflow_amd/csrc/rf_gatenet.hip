@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
 #include <cstdarg>
 #include <cstring>
 #include <deque>
@@ -150,12 +151,15 @@ __global__ void __launch_bounds__(256) gn_lane_hash(const uint32_t* __restrict__
 // power of two, so a wavefront of one gate covers 64 groups with one 512 B
 // access per operand.
 
+// A launch covers the first 2^le groups (those with lanes still running;
+// the stimuli are dealt to lanes longest budget first): thread t evaluates
+// gate t >> le for group t & (2^le - 1); planes keep their 2^lg-group stride.
 __global__ void __launch_bounds__(256) gn_bp_gates(const uint4* __restrict__ rec, uint32_t n,
-                                                   uint64_t* __restrict__ P, uint32_t lg) {
+                                                   uint64_t* __restrict__ P, uint32_t lg, uint32_t le) {
     const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    const uint32_t k = (uint32_t)(t >> lg);
+    const uint32_t k = (uint32_t)(t >> le);
     if (k >= n) return;
-    const uint32_t g = (uint32_t)t & ((1u << lg) - 1u);
+    const uint32_t g = (uint32_t)t & ((1u << le) - 1u);
     const uint4 r = rec[k];
     const uint32_t ty = r.x >> 29;
     const uint64_t a = P[((size_t)r.y << lg) + g], b = P[((size_t)r.z << lg) + g];
@@ -164,14 +168,18 @@ __global__ void __launch_bounds__(256) gn_bp_gates(const uint4* __restrict__ rec
 }
 
 // One wavefront per group: lane t's primary-input word, transposed into
-// planes (thread i ends with input bit i of the group's 64 lanes).
+// planes (thread i ends with input bit i of the group's 64 lanes); a lane
+// runs stimulus sid[lane].
 __global__ void __launch_bounds__(64) gn_bp_stim(uint64_t* __restrict__ P, uint32_t lg, uint32_t n_pi,
-                                                 const uint32_t* __restrict__ ctl) {
+                                                 const uint32_t* __restrict__ ctl,
+                                                 const uint32_t* __restrict__ sid, uint32_t nlanes) {
     const uint32_t g = blockIdx.x, t = threadIdx.x;
     const uint64_t seed = (uint64_t)ctl[2] | ((uint64_t)ctl[3] << 32);
     const uint32_t cyc = ctl[0];
+    const uint32_t lane = g * 64u + t;
+    const uint64_t s = lane < nlanes ? sid[lane] : lane;
     for (uint32_t j = 0; j * 64u < n_pi; ++j) {
-        const uint64_t col = bp_transpose64(rf_stim(seed, g * 64u + t, cyc, j), t);
+        const uint64_t col = bp_transpose64(rf_stim(seed, s, cyc, j), t);
         if (64u * j + t < n_pi) P[((size_t)(64u * j + t) << lg) + g] = col;
     }
 }
@@ -183,20 +191,21 @@ __global__ void __launch_bounds__(64) gn_bp_stim(uint64_t* __restrict__ P, uint3
 // Flip-flops read gates and write flip-flop nodes; RAM ports read gates and
 // write read-data nodes: no block reads what another writes.
 __global__ void __launch_bounds__(64) gn_bp_edge(const uint32_t* __restrict__ prog, uint64_t* __restrict__ P,
-                                                 uint32_t lg, uint32_t ffb, uint64_t* __restrict__ M) {
+                                                 uint32_t lg, uint32_t le, uint32_t ffb,
+                                                 uint64_t* __restrict__ M) {
     const uint32_t t = threadIdx.x;
     const uint32_t n_pi = prog[H_PI], n_ff = prog[H_FF];
-    const uint32_t G = 1u << lg;
+    const uint32_t E = 1u << le;     // groups this launch covers
     if (blockIdx.x < ffb) {
         const uint64_t i = (uint64_t)blockIdx.x * 64u + t;
-        const uint32_t f = (uint32_t)(i >> lg), g = (uint32_t)i & (G - 1u);
+        const uint32_t f = (uint32_t)(i >> le), g = (uint32_t)i & (E - 1u);
         if (f >= n_ff) return;
         const uint32_t d = prog[prog[H_OFF_FF] + f];
         P[((size_t)(n_pi + f) << lg) + g] = P[((size_t)d << lg) + g];
         return;
     }
     const uint32_t b = blockIdx.x - ffb;
-    const uint32_t g = b & (G - 1u), r = b >> lg;
+    const uint32_t g = b & (E - 1u), r = b >> le;
     const uint32_t nr = prog[H_RAMS], depth = prog[H_DEPTH], width = prog[H_WIDTH], ab = prog[H_ABITS];
     if (r >= nr) return;
     const uint32_t* rp = prog + prog[H_OFF_RAM] + r * (ab + 1u + width);
@@ -260,9 +269,14 @@ struct rf_gatenet {
     uint64_t* hash = nullptr;
     uint32_t* budget = nullptr;
     uint32_t* ctl = nullptr;
+    uint32_t* sid = nullptr;        // bit-plane form: stimulus of each lane
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     uint32_t nodes[2] = {0, 0};
+    // bit-plane cycle graphs over the first 2^e groups, e = 0 .. lg
+    hipGraph_t bgraph[32] = {};
+    hipGraphExec_t bexec[32] = {};
+    uint32_t bnodes[32] = {};
     hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 
@@ -287,15 +301,15 @@ int add_kernel(hipGraph_t gr, const void* fn, dim3 grid, dim3 block, void** args
 // Levels of records [lev[l], lev[l + 1]) at ``rec``: one bit-plane gate
 // kernel per level, chained after ``prev``.
 int add_bp_levels(rf_gatenet* g, hipGraph_t gr, const uint32_t* lev, uint32_t nlev, const uint4* rec,
-                  hipGraphNode_t* prev, uint32_t* count) {
+                  uint32_t le, hipGraphNode_t* prev, uint32_t* count) {
     for (uint32_t l = 0; l < nlev; ++l) {
         const uint32_t n = lev[l + 1] - lev[l];
         if (!n) continue;
         const uint4* r = rec + lev[l];
         uint64_t* P = g->P;
-        uint32_t nn = n, lg = g->lg;
-        void* args[] = {&r, &nn, &P, &lg};
-        const uint64_t threads = (uint64_t)n << g->lg;
+        uint32_t nn = n, lg = g->lg, e = le;
+        void* args[] = {&r, &nn, &P, &lg, &e};
+        const uint64_t threads = (uint64_t)n << le;
         hipGraphNode_t node;
         int rc = add_kernel(gr, (const void*)gn_bp_gates, dim3((uint32_t)((threads + 255) / 256)), dim3(256), args,
                             prev, 1, &node);
@@ -306,35 +320,37 @@ int add_bp_levels(rf_gatenet* g, hipGraph_t gr, const uint32_t* lev, uint32_t nl
     return RF_OK;
 }
 
-int build_bitplane_graph(rf_gatenet* g) {
+// The bit-plane cycle graph over the first E = 2^le groups.
+int build_bitplane_graph(rf_gatenet* g, uint32_t le) {
     const uint32_t* h = H(g);
     hipGraph_t gr;
     GN_HIP(hipGraphCreate(&gr, 0));
-    g->graph[RF_GN_BITPLANE] = gr;
-    const uint32_t G = 1u << g->lg;
+    g->bgraph[le] = gr;
+    const uint32_t G = 1u << le;
     uint32_t count = 0;
     hipGraphNode_t prev;
     {
         uint64_t* P = g->P;
-        uint32_t lg = g->lg, npi = h[H_PI];
+        uint32_t lg = g->lg, npi = h[H_PI], nl = g->nlanes;
         uint32_t* ctl = g->ctl;
-        void* args[] = {&P, &lg, &npi, &ctl};
+        const uint32_t* sid = g->sid;
+        void* args[] = {&P, &lg, &npi, &ctl, &sid, &nl};
         int rc = add_kernel(gr, (const void*)gn_bp_stim, dim3(G), dim3(64), args, nullptr, 0, &prev);
         if (rc) return rc;
         ++count;
     }
     const uint4* recf = reinterpret_cast<const uint4*>(g->prog + h[H_OFF_REC_FULL]);
     const uint4* recc = reinterpret_cast<const uint4*>(g->prog + h[H_OFF_REC_CONE]);
-    int rc = add_bp_levels(g, gr, h + h[H_OFF_LEV_FULL], h[H_LEV_FULL], recf, &prev, &count);
+    int rc = add_bp_levels(g, gr, h + h[H_OFF_LEV_FULL], h[H_LEV_FULL], recf, le, &prev, &count);
     if (rc) return rc;
     {
-        const uint32_t ffb = (uint32_t)((((uint64_t)h[H_FF] << g->lg) + 63) / 64);
+        const uint32_t ffb = (uint32_t)((((uint64_t)h[H_FF] << le) + 63) / 64);
         const uint32_t blocks = ffb + G * h[H_RAMS];
         const uint32_t* prog = g->prog;
         uint64_t* P = g->P;
-        uint32_t lg = g->lg, fb = ffb;
+        uint32_t lg = g->lg, e = le, fb = ffb;
         uint64_t* M = g->M;
-        void* args[] = {&prog, &P, &lg, &fb, &M};
+        void* args[] = {&prog, &P, &lg, &e, &fb, &M};
         hipGraphNode_t node;
         if (blocks) {
             rc = add_kernel(gr, (const void*)gn_bp_edge, dim3(blocks), dim3(64), args, &prev, 1, &node);
@@ -343,7 +359,7 @@ int build_bitplane_graph(rf_gatenet* g) {
             ++count;
         }
     }
-    rc = add_bp_levels(g, gr, h + h[H_OFF_LEV_CONE], h[H_LEV_CONE], recc, &prev, &count);
+    rc = add_bp_levels(g, gr, h + h[H_OFF_LEV_CONE], h[H_LEV_CONE], recc, le, &prev, &count);
     if (rc) return rc;
     {
         const uint32_t* prog = g->prog;
@@ -367,8 +383,8 @@ int build_bitplane_graph(rf_gatenet* g) {
         if (rc) return rc;
         ++count;
     }
-    GN_HIP(hipGraphInstantiate(&g->exec[RF_GN_BITPLANE], gr, nullptr, nullptr, 0));
-    g->nodes[RF_GN_BITPLANE] = count;
+    GN_HIP(hipGraphInstantiate(&g->bexec[le], gr, nullptr, nullptr, 0));
+    g->bnodes[le] = count;
     return RF_OK;
 }
 
@@ -538,7 +554,11 @@ int rf_gn_destroy(rf_gatenet* g) {
         if (g->exec[k]) (void)hipGraphExecDestroy(g->exec[k]);
         if (g->graph[k]) (void)hipGraphDestroy(g->graph[k]);
     }
-    void* ps[] = {g->prog, g->P, g->X, g->M, g->hash, g->budget, g->ctl};
+    for (int k = 0; k < 32; ++k) {
+        if (g->bexec[k]) (void)hipGraphExecDestroy(g->bexec[k]);
+        if (g->bgraph[k]) (void)hipGraphDestroy(g->bgraph[k]);
+    }
+    void* ps[] = {g->prog, g->P, g->X, g->M, g->hash, g->budget, g->ctl, g->sid};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (g->e0) (void)hipEventDestroy(g->e0);
@@ -576,6 +596,7 @@ int rf_gn_create(const uint32_t* program, size_t words, uint32_t n_lanes, rf_gat
     if (e == hipSuccess) e = hipMalloc(&g->M, mbytes);
     if (e == hipSuccess) e = hipMalloc(&g->hash, (size_t)n_lanes * 8);
     if (e == hipSuccess) e = hipMalloc(&g->budget, (size_t)n_lanes * 4);
+    if (e == hipSuccess) e = hipMalloc(&g->sid, (size_t)n_lanes * 4);
     if (e == hipSuccess) e = hipMalloc(&g->ctl, 64);
     if (e != hipSuccess) return cleanup(rf_set_error(RF_E_HIP, "rf_gn_create: %s", hipGetErrorString(e)));
     *out = g;
@@ -585,7 +606,8 @@ int rf_gn_create(const uint32_t* program, size_t words, uint32_t n_lanes, rf_gat
 uint32_t rf_gn_mtasks(const rf_gatenet* g) { return g ? H(g)[H_MTASKS] : 0; }
 
 uint32_t rf_gn_nodes_per_cycle(const rf_gatenet* g, uint32_t mode) {
-    return g && mode <= RF_GN_BITPLANE ? g->nodes[mode] : 0;
+    if (!g || mode > RF_GN_BITPLANE) return 0;
+    return mode == RF_GN_BITPLANE ? g->bnodes[g->lg] : g->nodes[mode];
 }
 
 int rf_gn_run(rf_gatenet* g, uint32_t mode, uint64_t seed, const uint32_t* budgets, uint64_t* out_hashes,
@@ -604,9 +626,37 @@ int rf_gn_run(rf_gatenet* g, uint32_t mode, uint64_t seed, const uint32_t* budge
         const size_t bytes = (size_t)h[H_NODES] * g->npad;
         GN_HIP(hipMalloc(&g->X, bytes));
     }
-    if (!g->exec[mode]) {
-        int rc = mode == RF_GN_BITPLANE ? build_bitplane_graph(g) : build_lane_graph(g);
+    if (mode == RF_GN_GRAPH && !g->exec[mode]) {
+        int rc = build_lane_graph(g);
         if (rc) return rc;
+    }
+    // bit-plane form: stimuli dealt to lanes longest budget first, so the
+    // groups with lanes still running at cycle c are a prefix of A(c)
+    // groups and cycle c runs the graph over the first 2^ceil(log2 A(c))
+    std::vector<uint32_t> lane_sid, lane_budget, le_of;
+    if (mode == RF_GN_BITPLANE) {
+        const uint32_t n = g->nlanes;
+        lane_sid.resize(n);
+        std::iota(lane_sid.begin(), lane_sid.end(), 0u);
+        std::stable_sort(lane_sid.begin(), lane_sid.end(),
+                         [&](uint32_t x, uint32_t y) { return budgets[x] > budgets[y]; });
+        lane_budget.resize(n);
+        for (uint32_t l = 0; l < n; ++l) lane_budget[l] = budgets[lane_sid[l]];
+        const uint32_t ng = (n + 63) / 64;
+        std::vector<uint32_t> gmax(ng, 0);
+        for (uint32_t l = 0; l < n; ++l) gmax[l / 64] = std::max(gmax[l / 64], lane_budget[l]);
+        le_of.resize(ncyc);
+        uint32_t active = ng;
+        for (uint32_t c = 0; c < ncyc; ++c) {
+            while (active > 1 && gmax[active - 1] <= c) --active;
+            uint32_t e = 0;
+            while ((1u << e) < active) ++e;
+            le_of[c] = e;
+            if (!g->bexec[e]) {
+                int rc = build_bitplane_graph(g, e);
+                if (rc) return rc;
+            }
+        }
     }
     // the reset state: zero nodes and RAMs, hash basis, cycle 0
     if (mode == RF_GN_BITPLANE)
@@ -615,17 +665,31 @@ int rf_gn_run(rf_gatenet* g, uint32_t mode, uint64_t seed, const uint32_t* budge
         GN_HIP(hipMemsetAsync(g->X, 0, (size_t)h[H_NODES] * g->npad, g->stream));
     const size_t lanes_alloc = (size_t)64 << g->lg;
     if (h[H_RAMS]) GN_HIP(hipMemsetAsync(g->M, 0, lanes_alloc * h[H_RAMS] * h[H_DEPTH] * 8, g->stream));
-    GN_HIP(hipMemcpyAsync(g->budget, budgets, (size_t)g->nlanes * 4, hipMemcpyHostToDevice, g->stream));
+    if (mode == RF_GN_BITPLANE) {
+        GN_HIP(hipMemcpyAsync(g->budget, lane_budget.data(), (size_t)g->nlanes * 4, hipMemcpyHostToDevice,
+                              g->stream));
+        GN_HIP(hipMemcpyAsync(g->sid, lane_sid.data(), (size_t)g->nlanes * 4, hipMemcpyHostToDevice, g->stream));
+    } else {
+        GN_HIP(hipMemcpyAsync(g->budget, budgets, (size_t)g->nlanes * 4, hipMemcpyHostToDevice, g->stream));
+    }
     hipLaunchKernelGGL(gn_reset, dim3((g->nlanes + 255) / 256), dim3(256), 0, g->stream, g->hash, g->nlanes,
                        g->ctl, seed);
     GN_HIP(hipGetLastError());
     GN_HIP(hipEventRecord(g->e0, g->stream));
-    for (uint32_t c = 0; c < ncyc; ++c) GN_HIP(hipGraphLaunch(g->exec[mode], g->stream));
+    for (uint32_t c = 0; c < ncyc; ++c)
+        GN_HIP(hipGraphLaunch(mode == RF_GN_BITPLANE ? g->bexec[le_of[c]] : g->exec[mode], g->stream));
     GN_HIP(hipEventRecord(g->e1, g->stream));
     GN_HIP(hipEventSynchronize(g->e1));
     if (ms) GN_HIP(hipEventElapsedTime(ms, g->e0, g->e1));
-    GN_HIP(hipMemcpyAsync(out_hashes, g->hash, (size_t)g->nlanes * 8, hipMemcpyDeviceToHost, g->stream));
-    GN_HIP(hipStreamSynchronize(g->stream));
+    if (mode == RF_GN_BITPLANE) {
+        std::vector<uint64_t> lh(g->nlanes);
+        GN_HIP(hipMemcpyAsync(lh.data(), g->hash, (size_t)g->nlanes * 8, hipMemcpyDeviceToHost, g->stream));
+        GN_HIP(hipStreamSynchronize(g->stream));
+        for (uint32_t l = 0; l < g->nlanes; ++l) out_hashes[lane_sid[l]] = lh[l];
+    } else {
+        GN_HIP(hipMemcpyAsync(out_hashes, g->hash, (size_t)g->nlanes * 8, hipMemcpyDeviceToHost, g->stream));
+        GN_HIP(hipStreamSynchronize(g->stream));
+    }
     return RF_OK;
 }
 

@@ -186,7 +186,7 @@ RF_EXPORT int rf_host_protocol(const rf_design* d, uint32_t nlanes, uint64_t see
     if (mode == 1) {
         rf_args x = m.a;
         x.ncycles = ncycles;
-        host_kernel f = (host_kernel)d->k_fused;
+        host_kernel f = (host_kernel)(x.in.data || x.out.data ? d->k_fused_io : d->k_fused);
         for (uint32_t l = 0; l < nlanes; ++l) f(x, l);
     } else {
         for (uint32_t c = 0; c < ncycles; ++c) {
@@ -247,7 +247,7 @@ RF_EXPORT int rf_host_queue(const rf_design* d, uint32_t nlanes, uint64_t seed, 
     x.order = ord.data();
     x.out_hash = out;
     x.nstim = nstim;
-    host_kernel f = (host_kernel)d->k_persistent;
+    host_kernel f = (host_kernel)(x.in.data || x.out.data ? d->k_persistent_io : d->k_persistent);
     for (uint32_t l = 0; l < nlanes; ++l) f(x, l);
     return m.flags[RF_FLAG_UNCONVERGED] ? -4 : m.flags[RF_FLAG_STREAM] ? -1 : 0;
 }

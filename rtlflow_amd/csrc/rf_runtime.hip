@@ -125,6 +125,10 @@ rf_args args_for(const rf_model* m, uint32_t phase) {
     return a;
 }
 
+// A stream or capture is set: the fused and persistent launches take the
+// kernels compiled with the stream code.
+bool streams_set(const rf_model* m) { return m->base.in.data || m->base.out.data; }
+
 int launch(rf_model* m, const void* fn, const rf_args& a) {
     rf_args copy = a;
     void* params[] = {&copy};
@@ -849,7 +853,7 @@ int rf_run_cycles(rf_model* m, uint32_t n, uint32_t mode) {
         if (mode == RF_MODE_FUSED) {
             rf_args a = args_for(m, RF_PHASE_EVAL);
             a.ncycles = chunk;
-            rc = launch(m, m->d->k_fused, a);
+            rc = launch(m, streams_set(m) ? m->d->k_fused_io : m->d->k_fused, a);
             if (rc) return rc;
         } else {
             rf_bp_params p = m->bp;
@@ -964,7 +968,7 @@ int rf_queue_step(rf_model* m, uint32_t max_stimuli, uint32_t mode, uint32_t* po
         a.out_hash = m->out_hash;
         a.nstim = n;
         RF_HIP(hipEventRecord(m->ev0, m->stream));
-        rc = launch(m, m->d->k_persistent, a);
+        rc = launch(m, streams_set(m) ? m->d->k_persistent_io : m->d->k_persistent, a);
         if (rc) return rc;
         RF_HIP(hipEventRecord(m->ev1, m->stream));
         RF_HIP(hipEventSynchronize(m->ev1));
@@ -1165,7 +1169,7 @@ static int run_queue(rf_model* m, uint64_t seed, const uint32_t* budgets, uint32
     a.out_hash = dout;
     a.nstim = nstim;
     RF_HIP(hipEventRecord(m->ev0, m->stream));
-    int rc = launch(m, m->d->k_persistent, a);
+    int rc = launch(m, streams_set(m) ? m->d->k_persistent_io : m->d->k_persistent, a);
     if (rc) return rc;
     RF_HIP(hipEventRecord(m->ev1, m->stream));
     RF_HIP(hipEventSynchronize(m->ev1));

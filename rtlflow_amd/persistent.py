@@ -61,7 +61,9 @@ def image_load_lines(comp, ind: str, lds=None) -> list:
     return out
 
 
-def emit_persistent(comp) -> str:
+def emit_persistent(comp, io: bool = False) -> str:
+    """The batched kernel; ``io``: the variant with the input-stream and
+    output-capture code (rf_k_persistent_io), kept out of the plain one."""
     d = comp.design
     gen = Gen(comp)
     gen.lds = lds_memories(comp)
@@ -101,10 +103,10 @@ def emit_persistent(comp) -> str:
         gen.emit(ln)
     _initial_body(comp, gen)
     gen.emit("uint64_t h = RF_FNV_BASIS;")
-    pf = _stim_prefetch(comp, gen, "(uint64_t)sid", "0ull", "budget != 0u", True)
+    pf = io and _stim_prefetch(comp, gen, "(uint64_t)sid", "0ull", "budget != 0u", True)
     gen.emit("for (uint32_t cyc = 0u; cyc < budget; ++cyc) {")
     gen.ind += 1
-    _stim_code(comp, gen, "(uint64_t)sid", "(uint64_t)cyc", pf)
+    _stim_code(comp, gen, "(uint64_t)sid", "(uint64_t)cyc", pf, io)
     if pf:
         _stim_prefetch(comp, gen, "(uint64_t)sid", "(uint64_t)cyc + 1u", "cyc + 1u < budget", False)
     gen.emit("#pragma unroll 1")
@@ -116,14 +118,17 @@ def emit_persistent(comp) -> str:
     _eval_loop_body_split(comp, gen, "cyc == 0u", cone)
     gen.ind -= 1
     gen.emit("}")
-    _hash_code(comp, gen, "h", "(uint64_t)sid", "(uint64_t)cyc")
+    if io:
+        _hash_code(comp, gen, "h", "(uint64_t)sid", "(uint64_t)cyc")
+    else:
+        _hash_code(comp, gen, "h")
     gen.ind -= 1
     gen.emit("}")
     gen.emit("a.out_hash[sid] = h;")
     gen.emit(f"q = {atomic(comp, 'add', '&a.flags[RF_FLAG_QUEUE]', '1u')};")
     gen.ind -= 1
     gen.emit("}")
-    lines = [kernel_header(comp, "rf_k_persistent")]
+    lines = [kernel_header(comp, "rf_k_persistent_io" if io else "rf_k_persistent")]
     lines += _lds_decl_lines(comp, gen.lds) + lds_state
     _prologue(comp, gen, lines, check_done=False)
     lines += gen.top_lines + gen.lines
