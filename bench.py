@@ -288,12 +288,16 @@ def bench_netlist(args, torch):
     # stimulus-cycle are the bit-packed state read and written once
     state_bits = man["bitplane"]["state_bits"]
     b_alg = 2 * ((state_bits + 7) // 8)
+    # (groups beyond the workgroup slots rotate through several launches of
+    # one full round each: a launch's stimulus-cycles are the timed steps'
+    # total over the launches)
     launches = max(1, st["bitplane_launches"])
     per_launch_ms = st["bitplane_ms"] / launches
     cyc_per_launch = min(cycles, args.bp_chunk)
-    bytes_per_launch = b_alg * lanes * cyc_per_launch
+    bytes_per_launch = b_alg * K * stim_cycles / launches
     achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
-    shape = {"lanes": lanes, "cycles_per_launch": cyc_per_launch, "bitplane_threads": nt}
+    shape = {"lanes": lanes, "cycles_per_launch": cyc_per_launch, "bitplane_threads": nt,
+             "launches_per_step": launches // K}
     traffic = None
     t = profile_json(f"{args.design}_bitplane_traffic.json")
     if t and all(t.get(k) == v for k, v in shape.items()):
@@ -332,7 +336,8 @@ def bench_netlist(args, torch):
         "config": {"workload": f"random netlist 20k gates / 2k flops / 64 PI / 64 PO / depth 40, "
                                f"{lanes} stimuli x {cycles} cycles",
                    "design": args.design, "stimuli": lanes, "cycles": cycles,
-                   "cycles_per_launch": cyc_per_launch, "form": "bitplane",
+                   "cycles_per_launch": cyc_per_launch, "launches_per_step": launches // K,
+                   "form": "bitplane",
                    "bitplane_threads": nt, "mtasks": man["mtasks"]},
         "forms": forms,
         "learned_partition": learned,
@@ -408,11 +413,13 @@ def bench_stream(args, torch, cd, man, lanes, nt, seed):
             st = m.stats()
             same = bool((m.hashes() == ref).all())
             per_ms = st["bitplane_ms"] / max(1, st["bitplane_launches"])
+            kern_s = st["bitplane_ms"] * 1e-3          # every launch of the timed runs
             b = 2 * ((man["bitplane"]["state_bits"] + 7) // 8) + (8 * nin if use_in else 0) + \
                 (8 * nout if use_cap else 0)
-            out[tag] = {"value": reps * lanes * sc / dt, "kernel_value": lanes * sc / (per_ms * 1e-3),
-                        "avg_launch_ms": per_ms, "alg_bytes_per_stimulus_cycle": b,
-                        "hbm_frac": b * lanes * sc / (per_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+            out[tag] = {"value": reps * lanes * sc / dt, "kernel_value": reps * lanes * sc / kern_s,
+                        "avg_launch_ms": per_ms, "launches": st["bitplane_launches"],
+                        "alg_bytes_per_stimulus_cycle": b,
+                        "hbm_frac": b * reps * lanes * sc / kern_s / (HBM_PEAK_GBS * 1e9),
                         "hashes_equal_generator_run": same}
             log(f"[bench] bitplane {tag}: {out[tag]['value']:.4g} stim-cycles/s "
                 f"(kernel {out[tag]['kernel_value']:.4g}, {per_ms:.2f} ms per launch), "

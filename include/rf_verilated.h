@@ -45,6 +45,14 @@ typedef struct rf_config {
     uint32_t lane_pad;        /* SoA padding: extra lanes in every slot's row
                                  beyond n_lanes rounded up to the block
                                  (a multiple of 64; 0 = none) */
+    uint32_t bitplane_slots;  /* bit-plane form, rf_run_cycles: 64-lane
+                                 groups one launch runs at once. 0 = the
+                                 workgroups the GPU holds at once, and groups
+                                 beyond them rotate through launches of one
+                                 full round each when that is estimated
+                                 faster than a launch of several rounds;
+                                 N > 0 = rotate through windows of at most N
+                                 groups whenever there are more than N */
 } rf_config;
 
 typedef struct rf_model rf_model;

@@ -47,6 +47,15 @@ struct rf_bp_params {
     // chunks) when the launch holds them after the planes; 0: read from
     // ``prog``. Set when the extra LDS leaves the workgroups per CU unchanged.
     uint32_t tab_off;
+    // group rotation: with grp_count != 0 the launch runs grp_n workgroups,
+    // workgroup b serving the 64-lane group (grp_base + b) mod grp_count for
+    // a.ncycles cycles when b < grp_split, else for ncyc_hi cycles, and
+    // writing back every stored variable (program F) when fwb_lo (b <
+    // grp_split) or fwb_hi (the others) is set, else only the state; with
+    // grp_count == 0 workgroup b serves group b for a.ncycles cycles and
+    // writes back everything
+    uint32_t grp_base, grp_count, grp_n, grp_split, ncyc_hi;
+    uint32_t fwb_lo, fwb_hi;
 };
 
 // Parse and check a blob (every slot index, level table and section size);
@@ -58,6 +67,10 @@ int rf_bp_parse(const uint32_t* blob, uint64_t words, rf_bp_params* p, const cha
 // workgroup; nr = 0 when it needs more than RF_BP_MAXR rounds.
 void rf_bp_rounds(const uint32_t* blob, uint32_t threads, rf_bp_params* p);
 
-// Launch one kernel over ceil(nlanes / 64) workgroups of ``threads``
-// threads (128, 256, 512 or 1024).
+// Launch one kernel over ceil(nlanes / 64) workgroups (grp_n with group
+// rotation) of ``threads`` threads (128, 256, 512 or 1024).
 hipError_t rf_bp_launch(const rf_bp_params& p, uint32_t threads, hipStream_t stream);
+
+// Workgroups of the kernel instance rf_bp_launch would pick for ``p`` that
+// one CU holds at once (LDS, registers and threads), in *per_cu.
+hipError_t rf_bp_occupancy(const rf_bp_params& p, uint32_t threads, int* per_cu);

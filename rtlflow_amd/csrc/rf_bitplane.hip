@@ -40,6 +40,82 @@ __device__ __forceinline__ uint64_t bp_gather(const uint64_t* L, TP refs, uint32
     const uint64_t row = t < nbits ? bp_ref(L, refs[t]) : 0ull;
     return bp_transpose64(row, t);
 }
+// Low bits of 16 bytes (each 0 or 1) as 16 bits, byte i to bit i: per
+// 32-bit word, the multiply moves bit 8i to bit 24 + i without carries.
+__device__ __forceinline__ uint32_t bp_pack16(uint4 v) {
+    const auto p4 = [](uint32_t w) { return ((w & 0x01010101u) * 0x01020408u) >> 24; };
+    return p4(v.x) | (p4(v.y) << 4) | (p4(v.z) << 8) | (p4(v.w) << 12);
+}
+// 16 bits as 16 bytes of 0 or 1, bit i to byte i: per 4 bits, the multiply
+// moves bit i to bit 8i without carries.
+__device__ __forceinline__ uint4 bp_unpack16(uint32_t b) {
+    const auto u4 = [](uint32_t x) { return ((x & 15u) * 0x00204081u) & 0x01010101u; };
+    return make_uint4(u4(b), u4(b >> 4), u4(b >> 8), u4(b >> 12));
+}
+// Single-bit variables (class C, one byte per lane) of a 64-lane group,
+// four threads per variable, each moving its 16 lanes' bytes with one
+// 16-byte access: from the lane-minor array into planes (load), or from
+// planes back (store). ``tab`` holds 6-word entries (loc in words 1-2, the
+// plane's slot or reference in word 5); lanes outside ``lanes`` (the load:
+// lanes that exist, the store: lanes that write back) are zero in the
+// planes, resp. left untouched in memory. U entries per thread are in
+// flight at once.
+constexpr uint32_t kBpU = 4;
+template <int NT>
+__device__ __forceinline__ void bp_load_bits(uint64_t* L, const rf_args& a, const uint32_t* tab, uint32_t n,
+                                             uint32_t grp, uint64_t lanes, uint32_t tid) {
+    constexpr uint32_t Q = NT / 4;
+    const uint32_t q = tid & 3u;
+    const size_t col = (size_t)grp * 64u + 16u * q;
+    const uint32_t qm = (uint32_t)(lanes >> (16u * q)) & 0xffffu;
+    for (uint32_t k0 = tid >> 2; k0 < n; k0 += kBpU * Q) {
+        uint64_t loc[kBpU];
+        uint32_t sl[kBpU];
+#pragma unroll
+        for (uint32_t j = 0; j < kBpU; ++j) {
+            const uint32_t* e = tab + 6u * min(k0 + j * Q, n - 1u);
+            loc[j] = (uint64_t)e[1] | ((uint64_t)e[2] << 32);
+            sl[j] = e[5];
+        }
+        uint4 v[kBpU];
+#pragma unroll
+        for (uint32_t j = 0; j < kBpU; ++j) v[j] = *reinterpret_cast<const uint4*>(a.c + loc[j] * a.npad + col);
+#pragma unroll
+        for (uint32_t j = 0; j < kBpU; ++j)
+            if (k0 + j * Q < n) reinterpret_cast<uint16_t*>(L + sl[j])[q] = (uint16_t)(bp_pack16(v[j]) & qm);
+    }
+}
+template <int NT>
+__device__ __forceinline__ void bp_store_bits(const uint64_t* L, const rf_args& a, const uint32_t* tab, uint32_t n,
+                                              uint32_t grp, uint64_t lanes, uint32_t tid) {
+    constexpr uint32_t Q = NT / 4;
+    const uint32_t q = tid & 3u;
+    const size_t col = (size_t)grp * 64u + 16u * q;
+    const uint32_t qm = (uint32_t)(lanes >> (16u * q)) & 0xffffu;
+    if (!qm) return;
+    for (uint32_t k0 = tid >> 2; k0 < n; k0 += kBpU * Q) {
+        uint64_t loc[kBpU];
+        uint32_t ref[kBpU];
+#pragma unroll
+        for (uint32_t j = 0; j < kBpU; ++j) {
+            const uint32_t* e = tab + 6u * min(k0 + j * Q, n - 1u);
+            loc[j] = (uint64_t)e[1] | ((uint64_t)e[2] << 32);
+            ref[j] = e[5];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kBpU; ++j) {
+            if (k0 + j * Q >= n) continue;
+            const uint32_t b = (uint32_t)(bp_ref(L, ref[j]) >> (16u * q)) & 0xffffu;
+            uint8_t* dst = a.c + loc[j] * a.npad + col;
+            if (qm == 0xffffu) {
+                *reinterpret_cast<uint4*>(dst) = bp_unpack16(b);
+            } else {
+                for (uint32_t i = 0; i < 16u; ++i)
+                    if ((qm >> i) & 1u) dst[i] = (uint8_t)((b >> i) & 1u);
+            }
+        }
+    }
+}
 
 // Gate record, slot form: x = dst | s << 16; y = xs | ix << 15 | ys << 16 |
 // iy << 31. Byte form (BA, programs of at most 8,192 slots): every field is
@@ -394,7 +470,11 @@ rf_bp_kernel(rf_bp_params P) {
     const uint32_t t = tid & 63u;
     const bool w0 = tid < 64u;
     const bool w1 = tid >= 64u && tid < 128u;
-    const uint32_t lane = blockIdx.x * 64u + t;
+    // the 64-lane group this workgroup serves and its cycles this launch
+    const bool rot = P.grp_count != 0u;
+    const uint32_t grp = rot ? (blockIdx.x + P.grp_base) % P.grp_count : blockIdx.x;
+    const uint32_t ncyc = (rot && blockIdx.x >= P.grp_split) ? P.ncyc_hi : a.ncycles;
+    const uint32_t lane = grp * 64u + t;
     const bool inb = lane < a.nlanes;
     const uint32_t* prog = P.prog;
     const bool act = inb && a.done[lane] == 0;
@@ -407,47 +487,27 @@ rf_bp_kernel(rf_bp_params P) {
     }
     constexpr uint32_t NW = NT / 64;
     const uint32_t wave = tid >> 6;
-    // cycles this lane advances: a.ncycles, or with per-stimulus budgets
+    // cycles this lane advances: ncyc, or with per-stimulus budgets
     // (a.budget) what is left of its budget; the group runs as many cycles
     // as its longest lane needs (every wavefront reduces the same 64 lanes,
     // so no LDS word is needed and the planes start at LDS offset 0)
     const bool bud = a.budget != nullptr;
-    uint32_t lim = a.ncycles;
-    uint32_t ngroup = a.ncycles;
+    uint32_t lim = ncyc;
+    uint32_t ngroup = ncyc;
     if (bud) {
         const uint32_t c0 = inb ? a.cyc[lane] : 0u;
         const uint32_t b = act ? a.budget[a.stim[lane]] : 0u;
-        lim = b > c0 ? min(a.ncycles, b - c0) : 0u;
+        lim = b > c0 ? min(ncyc, b - c0) : 0u;
         ngroup = lim;
         for (uint32_t k = 32; k; k >>= 1) ngroup = max(ngroup, (uint32_t)__shfl_xor((int)ngroup, (int)k, 64));
     }
     if (tid == 0) L[0] = 0;
-    // state planes from the lane-minor arrays, spread over all wavefronts
-    // (lane t of the group is thread t of every wavefront): single-bit
-    // variables by one ballot each, wider ones by transposes
+    const uint32_t* st = prog + P.off_state;
+    // state planes from the lane-minor arrays, spread over all threads:
+    // single-bit variables four threads each (the lanes that exist: lane t
+    // of the group is thread t of every wavefront), wider ones by transposes
     {
-        const uint32_t* st = prog + P.off_state;
-        // batches of 16 single-bit entries (class C: one byte per lane),
-        // round-robin over the wavefronts; the batch's loads are issued back
-        // to back, so one memory latency covers the batch
-        constexpr uint32_t SB = 16;
-        for (uint32_t k0 = wave * SB; k0 < P.n_state1; k0 += NW * SB) {
-            const uint32_t cnt = min(SB, P.n_state1 - k0);
-            uint32_t v[SB];
-#pragma unroll
-            for (uint32_t j = 0; j < SB; ++j) {
-                const uint32_t* e = st + 6u * (k0 + min(j, cnt - 1u));
-                const uint64_t loc = (uint64_t)e[1] | ((uint64_t)e[2] << 32);
-                v[j] = inb ? a.c[(size_t)loc * a.npad + lane] : 0u;
-            }
-#pragma unroll
-            for (uint32_t j = 0; j < SB; ++j) {
-                if (j < cnt) {
-                    const uint64_t pl = __ballot((int)(v[j] & 1u));
-                    if (t == j) L[st[6u * (k0 + j) + 5]] = pl;
-                }
-            }
-        }
+        bp_load_bits<NT>(L, a, st, P.n_state1, grp, __ballot((int)inb), tid);
         for (uint32_t k2 = P.n_state1 + wave; k2 < P.n_state; k2 += NW) {
             const uint32_t* sl = st + 6u * k2;
             const uint32_t cls = sl[0], words = sl[3], width = sl[4], s0 = sl[5];
@@ -465,7 +525,7 @@ rf_bp_kernel(rf_bp_params P) {
         }
     }
     const uint32_t* in_tab = prog + P.off_in;
-    if (w0 && a.ncycles) bp_inputs<IO>(L, in_tab, P.n_in, a, sid, cyc, t, act && lim > 0);
+    if (w0 && ncyc) bp_inputs<IO>(L, in_tab, P.n_in, a, sid, cyc, t, act && lim > 0);
     // per-cycle tables into LDS after the planes
     const lds_u32* tc = reinterpret_cast<const lds_u32*>((size_t)P.tab_off);
     const lds_u32* th = tc + P.n_commit[1];
@@ -558,7 +618,28 @@ rf_bp_kernel(rf_bp_params P) {
     // stopped at different cycles: no state is written back (the hashes are
     // the result).
     const uint32_t* fc = prog + P.off_f;
-    const uint32_t nf = bud ? 0u : P.n_f;
+    // with group rotation, a group's runs before its last write back only
+    // the state (the next run loads nothing else)
+    const bool full_wb = !rot || (blockIdx.x < P.grp_split ? P.fwb_lo : P.fwb_hi);
+    const uint32_t nf = (bud || !full_wb) ? 0u : P.n_f;
+    if (!bud && !full_wb) {
+        bp_store_bits<NT>(L, a, st, P.n_state1, grp, __ballot((int)act), tid);
+        for (uint32_t k2 = P.n_state1 + wave; k2 < P.n_state; k2 += NW) {
+            const uint32_t* sl = st + 6u * k2;
+            const uint32_t cls = sl[0], words = sl[3], width = sl[4], s0 = sl[5];
+            const uint64_t loc = (uint64_t)sl[1] | ((uint64_t)sl[2] << 32);
+            if (width <= 64) {
+                const uint64_t v = bp_transpose64(t < width ? L[s0 + t] : 0ull, t);
+                if (act) bp_store(a, cls, loc, lane, v);
+            } else {
+                for (uint32_t w = 0; w < words; ++w) {
+                    const uint32_t nb = min(32u, width - 32u * w);
+                    const uint64_t v = bp_transpose64(t < nb ? L[s0 + 32u * w + t] : 0ull, t);
+                    if (act) a.i[(size_t)(loc + w) * a.npad + lane] = (uint32_t)v;
+                }
+            }
+        }
+    }
     for (uint32_t c = 0; c < nf; ++c) {
         const uint32_t fnl = fc[0], fng = fc[1], nwb = fc[2], nwb1 = fc[3];
         const uint32_t* flev = fc + RF_BP_FCHUNK_HDR;
@@ -566,17 +647,8 @@ rf_bp_kernel(rf_bp_params P) {
         go += go & 1u;
         bp_phase<NT, BA>(L, flev, reinterpret_cast<const uint2*>(fc + go), fnl, tid);
         const uint32_t* wp = fc + go + 2u * fng;
-        // single-bit entries (class C), 8 per wavefront step
-        for (uint32_t k0 = wave * 8u; k0 < nwb1; k0 += NW * 8u) {
-#pragma unroll
-            for (uint32_t j = 0; j < 8u; ++j) {
-                if (k0 + j < nwb1) {
-                    const uint32_t* e = wp + 6u * (k0 + j);
-                    const uint64_t v = (bp_ref(L, e[5]) >> t) & 1ull;
-                    if (act) a.c[(size_t)((uint64_t)e[1] | ((uint64_t)e[2] << 32)) * a.npad + lane] = (uint8_t)v;
-                }
-            }
-        }
+        // single-bit entries (class C), four threads each
+        bp_store_bits<NT>(L, a, wp, nwb1, grp, __ballot((int)act), tid);
         wp += 6u * nwb1;
         for (uint32_t k = nwb1; k < nwb; ++k) {
             const uint32_t cls = wp[0], words = wp[3], width = wp[4];
@@ -825,17 +897,20 @@ static uint32_t bp_tab_words(const rf_bp_params& p) {
     return p.n_commit[1] + p.hash_words + 4u * p.n_in;
 }
 
+// Launches the instance, or with ``occ`` set only reports how many of its
+// workgroups one CU holds at once.
 template <int NT, int MAXR, bool BA, bool IO>
-static hipError_t bp_launch_io(const rf_bp_params& p, hipStream_t stream) {
+static hipError_t bp_launch_io(const rf_bp_params& p, hipStream_t stream, int* occ) {
     const size_t shmem = (size_t)p.nslots * 8u + (p.tab_off ? 4u * (size_t)bp_tab_words(p) : 0u);
+    const void* fn = reinterpret_cast<const void*>(&rf_bp_kernel<NT, MAXR, BA, IO>);
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rf_bp_kernel<NT, MAXR, BA, IO>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    const uint32_t grid = (p.a.nlanes + 63u) / 64u;
+    if (occ) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, fn, NT, shmem);
+    const uint32_t grid = p.grp_count ? p.grp_n : (p.a.nlanes + 63u) / 64u;
     hipLaunchKernelGGL((rf_bp_kernel<NT, MAXR, BA, IO>), dim3(grid), dim3(NT), shmem, stream, p);
     return hipGetLastError();
 }
@@ -844,33 +919,47 @@ static hipError_t bp_launch_io(const rf_bp_params& p, hipStream_t stream) {
 // output capture: its extra registers cost the generator-only launches
 // spills in the cycle loop.
 template <int NT, int MAXR, bool BA>
-static hipError_t bp_launch_ba(const rf_bp_params& p, hipStream_t stream) {
-    return (p.a.in.data || p.a.out.data) ? bp_launch_io<NT, MAXR, BA, true>(p, stream)
-                                         : bp_launch_io<NT, MAXR, BA, false>(p, stream);
+static hipError_t bp_launch_ba(const rf_bp_params& p, hipStream_t stream, int* occ) {
+    return (p.a.in.data || p.a.out.data) ? bp_launch_io<NT, MAXR, BA, true>(p, stream, occ)
+                                         : bp_launch_io<NT, MAXR, BA, false>(p, stream, occ);
 }
 
 template <int NT, int MAXR>
-static hipError_t bp_launch_k(const rf_bp_params& p, hipStream_t stream) {
-    return p.byte_addr ? bp_launch_ba<NT, MAXR, true>(p, stream) : bp_launch_ba<NT, MAXR, false>(p, stream);
+static hipError_t bp_launch_k(const rf_bp_params& p, hipStream_t stream, int* occ) {
+    return p.byte_addr ? bp_launch_ba<NT, MAXR, true>(p, stream, occ)
+                       : bp_launch_ba<NT, MAXR, false>(p, stream, occ);
 }
 
 template <int NT>
-static hipError_t bp_launch_nt(const rf_bp_params& p, hipStream_t stream) {
+static hipError_t bp_launch_nt(const rf_bp_params& p, hipStream_t stream, int* occ) {
     const uint32_t r = p.nr;
-    if (r == 0 || r > RF_BP_MAXR || (NT >= 1024 && r > 40)) return bp_launch_k<NT, 0>(p, stream);
-    if (r <= 16) return bp_launch_k<NT, 16>(p, stream);
-    if (r <= 24) return bp_launch_k<NT, 24>(p, stream);
-    if (r <= 32) return bp_launch_k<NT, 32>(p, stream);
-    if (NT >= 1024 || r <= 40) return bp_launch_k<NT, 40>(p, stream);
-    return bp_launch_k<NT, (NT >= 1024 ? 40 : 48)>(p, stream);
+    if (r == 0 || r > RF_BP_MAXR || (NT >= 1024 && r > 40)) return bp_launch_k<NT, 0>(p, stream, occ);
+    if (r <= 16) return bp_launch_k<NT, 16>(p, stream, occ);
+    if (r <= 24) return bp_launch_k<NT, 24>(p, stream, occ);
+    if (r <= 32) return bp_launch_k<NT, 32>(p, stream, occ);
+    if (NT >= 1024 || r <= 40) return bp_launch_k<NT, 40>(p, stream, occ);
+    return bp_launch_k<NT, (NT >= 1024 ? 40 : 48)>(p, stream, occ);
+}
+
+static hipError_t bp_dispatch(const rf_bp_params& p, uint32_t threads, hipStream_t stream, int* occ) {
+    switch (threads) {
+        case 128: return bp_launch_k<128, 0>(p, stream, occ);
+        case 256: return bp_launch_nt<256>(p, stream, occ);
+        case 512: return bp_launch_nt<512>(p, stream, occ);
+        case 1024: return bp_launch_nt<1024>(p, stream, occ);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t rf_bp_launch(const rf_bp_params& p, uint32_t threads, hipStream_t stream) {
-    switch (threads) {
-        case 128: return bp_launch_k<128, 0>(p, stream);
-        case 256: return bp_launch_nt<256>(p, stream);
-        case 512: return bp_launch_nt<512>(p, stream);
-        case 1024: return bp_launch_nt<1024>(p, stream);
-        default: return hipErrorInvalidValue;
-    }
+    if (p.grp_count && (p.grp_n == 0 || p.grp_n > p.grp_count || p.grp_base >= p.grp_count ||
+                        p.grp_count != (p.a.nlanes + 63u) / 64u))
+        return hipErrorInvalidValue;
+    return bp_dispatch(p, threads, stream, nullptr);
+}
+
+hipError_t rf_bp_occupancy(const rf_bp_params& p, uint32_t threads, int* per_cu) {
+    if (!per_cu) return hipErrorInvalidValue;
+    *per_cu = 0;
+    return bp_dispatch(p, threads, nullptr, per_cu);
 }

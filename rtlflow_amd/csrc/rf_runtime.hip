@@ -71,6 +71,7 @@ struct rf_model {
     uint32_t grid = 0;
     hipStream_t stream = nullptr;
     int device = 0;
+    int ncu = 0;                       // compute units of the device
     uint8_t* queue_scratch = nullptr;  // this device's rf_queue_scratch (rf_submit_queue_static)
     int queue_scratch_err = 0;         // hipGetSymbolAddress's error, if it failed
     void* cls[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -541,6 +542,7 @@ int rf_create(const rf_design* d, uint32_t n_lanes, const rf_config* cfg, rf_mod
         if (cfg->fused_chunk) m->cfg.fused_chunk = cfg->fused_chunk;
         if (cfg->bitplane_threads) m->cfg.bitplane_threads = cfg->bitplane_threads;
         m->cfg.lane_pad = cfg->lane_pad;
+        m->cfg.bitplane_slots = cfg->bitplane_slots;
     }
     m->nlanes = n_lanes;
     const uint64_t q = d->block > 64 ? d->block : 64;
@@ -552,6 +554,8 @@ int rf_create(const rf_design* d, uint32_t n_lanes, const rf_config* cfg, rf_mod
     };
     hipError_t e = hipGetDevice(&m->device);
     if (e != hipSuccess) return cleanup(fail(RF_E_HIP, "hipGetDevice: %s", hipGetErrorString(e)));
+    e = hipDeviceGetAttribute(&m->ncu, hipDeviceAttributeMultiprocessorCount, m->device);
+    if (e != hipSuccess) return cleanup(fail(RF_E_HIP, "hipDeviceGetAttribute: %s", hipGetErrorString(e)));
     m->queue_scratch_err = (int)queue_scratch_for(m->device, &m->queue_scratch);
     // the bookkeeping of queued launches never grows past these, so waiting
     // for them (rf_submit_queue_static included) allocates nothing
@@ -826,6 +830,56 @@ int rf_begin(rf_model* m, uint64_t seed, uint32_t stim0) {
     return RF_OK;
 }
 
+// One launch of the bit-plane group rotation: a window of n consecutive
+// groups (cyclically) from group base; the first split of them run c_lo
+// cycles, the rest c_hi; f_lo / f_hi: those groups run for the last time
+// and write back every stored variable (otherwise only their state).
+struct bp_window {
+    uint32_t base, n, split, c_lo, c_hi, f_lo, f_hi;
+};
+
+// Plan of the bit-plane group rotation for G groups of which S run at once,
+// advancing every group n cycles. Each group runs in r of the launches
+// (n / r cycles each, the first n % r of its runs one cycle more): the
+// launches take consecutive windows of S positions of the sequence that
+// lists groups 0..G-1 r times, so all but the last launch are full rounds
+// and a group's runs come in order, each in a later launch. r is chosen by
+// an estimate in cycle units of a full round: a launch costs its cycles
+// plus a fixed 32 (state load, first and last programs, write-back); a
+// single launch over several rounds costs n for the first round and 0.7 n
+// for each further one (later rounds hold fewer workgroups per CU and run
+// faster per cycle). forced: rotate whenever G > S. An empty plan means one
+// launch over all groups.
+static std::vector<bp_window> bp_rotation(uint32_t G, uint32_t S, uint32_t n, bool forced) {
+    std::vector<bp_window> plan;
+    if (!S || G <= S || !n) return plan;
+    constexpr double kFixed = 32.0;
+    const uint32_t rounds = (G + S - 1) / S;
+    double best = forced ? 1e300 : n * (1.0 + 0.7 * (rounds - 1)) + kFixed;
+    uint32_t br = 0;
+    for (uint32_t r = 1; r <= 8 && r <= n; ++r) {
+        const uint64_t launches = ((uint64_t)r * G + S - 1) / S;
+        const double cost = (double)launches * ((n + r - 1) / r + kFixed);
+        if (cost < best) best = cost, br = r;
+    }
+    if (!br) return plan;
+    const uint64_t total = (uint64_t)br * G;
+    auto cyc = [&](uint64_t j) { return n / br + (j < n % br ? 1u : 0u); };
+    for (uint64_t q0 = 0; q0 < total; q0 += S) {
+        bp_window w;
+        w.n = (uint32_t)std::min<uint64_t>(S, total - q0);
+        const uint64_t j0 = q0 / G;
+        w.base = (uint32_t)(q0 % G);
+        w.split = (uint32_t)std::min<uint64_t>(w.n, (j0 + 1) * G - q0);
+        w.c_lo = cyc(j0);
+        w.c_hi = j0 + 1 < br ? cyc(j0 + 1) : 0u;
+        w.f_lo = j0 + 1 == br;
+        w.f_hi = j0 + 2 == br;
+        plan.push_back(w);
+    }
+    return plan;
+}
+
 int rf_run_cycles(rf_model* m, uint32_t n, uint32_t mode) {
     if (!m) return fail(RF_E_ARG, "null model");
     if (mode > RF_MODE_BITPLANE) return fail(RF_E_ARG, "unknown run mode %u", mode);
@@ -855,14 +909,41 @@ int rf_run_cycles(rf_model* m, uint32_t n, uint32_t mode) {
             a.ncycles = chunk;
             rc = launch(m, streams_set(m) ? m->d->k_fused_io : m->d->k_fused, a);
             if (rc) return rc;
-        } else {
+        }
+        uint32_t launches = 1;
+        if (mode == RF_MODE_BITPLANE) {
             rf_bp_params p = m->bp;
             rf_bp_rounds(m->d->bitplane, m->cfg.bitplane_threads, &p);
             p.a = args_for(m, RF_PHASE_EVAL);
             p.a.ncycles = chunk;
-            RF_HIP(rf_bp_launch(p, m->cfg.bitplane_threads, m->stream));
+            // groups beyond what the GPU holds at once rotate through
+            // launches of one full round each
+            const uint32_t G = (m->nlanes + 63u) / 64u;
+            uint32_t S = m->cfg.bitplane_slots;
+            if (!S) {
+                int per_cu = 0;
+                RF_HIP(rf_bp_occupancy(p, m->cfg.bitplane_threads, &per_cu));
+                S = (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, m->ncu);
+            }
+            const std::vector<bp_window> plan = bp_rotation(G, S, chunk, m->cfg.bitplane_slots != 0);
+            if (plan.empty()) {
+                RF_HIP(rf_bp_launch(p, m->cfg.bitplane_threads, m->stream));
+            } else {
+                p.grp_count = G;
+                for (const bp_window& w : plan) {
+                    p.grp_base = w.base;
+                    p.grp_n = w.n;
+                    p.grp_split = w.split;
+                    p.a.ncycles = w.c_lo;
+                    p.ncyc_hi = w.c_hi;
+                    p.fwb_lo = w.f_lo;
+                    p.fwb_hi = w.f_hi;
+                    RF_HIP(rf_bp_launch(p, m->cfg.bitplane_threads, m->stream));
+                }
+                launches = (uint32_t)plan.size();
+            }
         }
-        rc = timed_end(m, e0, mode == RF_MODE_FUSED ? KIND_FUSED : KIND_BITPLANE, 1);
+        rc = timed_end(m, e0, mode == RF_MODE_FUSED ? KIND_FUSED : KIND_BITPLANE, launches);
         if (rc) return rc;
         left -= chunk;
     }
