@@ -56,6 +56,13 @@ struct rf_bp_params {
     // writes back everything
     uint32_t grp_base, grp_count, grp_n, grp_split, ncyc_hi;
     uint32_t fwb_lo, fwb_hi;
+    // progress balancing: every 8 cycles a workgroup adds the cycles it has
+    // run to *progress (zeroed before the launch) and, while it has run
+    // fewer than the launch's average, issues at a raised wave priority;
+    // the decision goes to all its wavefronts through the LDS word at byte
+    // offset bal_off. Off when progress is null or bal_off is 0.
+    unsigned long long* progress;
+    uint32_t bal_off;
 };
 
 // Parse and check a blob (every slot index, level table and section size);

@@ -13,6 +13,8 @@
 //
 // which covers AND, OR, XOR, NAND, NOR, XNOR, NOT and 2:1 MUX with the
 // inversions folded into ix / iy.
+#include <algorithm>
+
 #include "rf_bitplane.h"
 #include "rf_stim.h"
 #include "rf_transpose.h"
@@ -574,7 +576,15 @@ rf_bp_kernel(rf_bp_params P) {
         }
         __syncthreads();
         // cycle n's outputs and cycle n + 1's next state in one pass
+        const bool bal = P.progress != nullptr && P.bal_off != 0u;
+        lds_u32* balw = reinterpret_cast<lds_u32*>((size_t)P.bal_off);
         for (uint32_t n = 0; n + 1 < N; ++n) {
+            if (bal && (n & 7u) == 0u && n) {
+                // behind the launch's average: issue ahead of the CU's
+                // other workgroups (the oldest workgroup wins otherwise)
+                if (*balw) __builtin_amdgcn_s_setprio(3);
+                else __builtin_amdgcn_s_setprio(0);
+            }
             if constexpr (MAXR > 0)
                 bp_rounds<MR, BA>(L, d, P);
             else
@@ -591,6 +601,9 @@ rf_bp_kernel(rf_bp_params P) {
                                                             cyc + n);
                 if (act && n < lim) h = hn;
             } else if (w0) {
+                const bool tick = bal && (n & 7u) == 7u && t == 0u;
+                unsigned long long done_all = 0;
+                if (tick) done_all = atomicAdd(P.progress, 8ull) + 8ull;
                 if (P.tab_off) {
                     bp_copy_inputs(L, ti, P.n_in, t);
                     if (n + 2 < N) bp_inputs<IO>(L, ti, P.n_in, a, sid, cyc + n + 2, t, act && n + 2 < lim);
@@ -598,9 +611,11 @@ rf_bp_kernel(rf_bp_params P) {
                     bp_copy_inputs(L, in_tab, P.n_in, t);
                     if (n + 2 < N) bp_inputs<IO>(L, in_tab, P.n_in, a, sid, cyc + n + 2, t, act && n + 2 < lim);
                 }
+                if (tick) *balw = (unsigned long long)(n + 1u) * gridDim.x < done_all ? 1u : 0u;
             }
             __syncthreads();
         }
+        if (bal) __builtin_amdgcn_s_setprio(0);
         // the last cycle's outputs
         bp_phase<NT, BA>(L, lev[RF_BP_B], gates[RF_BP_B], P.nlev[RF_BP_B], tid);
         if (w1) {
@@ -866,7 +881,11 @@ int rf_bp_parse(const uint32_t* b, uint64_t words, rf_bp_params* p, const char**
     const uint64_t planes = (uint64_t)p->nslots * 8u;
     const uint64_t tab = 4ull * (p->n_commit[1] + (uint64_t)p->hash_words + 4ull * p->n_in);
     const uint64_t cu = 160u * 1024u;
-    p->tab_off = (planes + tab <= cu - 64u && cu / planes == cu / (planes + tab)) ? (uint32_t)planes : 0u;
+    p->tab_off = (planes + tab + 8u <= cu - 64u && cu / planes == cu / (planes + tab + 8u)) ? (uint32_t)planes : 0u;
+    // the progress-balancing word after them, when it costs no workgroup per CU
+    const uint64_t used = planes + (p->tab_off ? tab : 0u);
+    const uint64_t bal = (used + 7u) & ~7ull;
+    p->bal_off = (bal + 8u <= cu - 64u && cu / planes == cu / (bal + 8u)) ? (uint32_t)bal : 0u;
     return 0;
 #undef BP_FAIL
 }
@@ -897,11 +916,16 @@ static uint32_t bp_tab_words(const rf_bp_params& p) {
     return p.n_commit[1] + p.hash_words + 4u * p.n_in;
 }
 
+static size_t bp_shmem(const rf_bp_params& p) {
+    const size_t base = (size_t)p.nslots * 8u + (p.tab_off ? 4u * (size_t)bp_tab_words(p) : 0u);
+    return p.bal_off ? std::max(base, (size_t)p.bal_off + 8u) : base;
+}
+
 // Launches the instance, or with ``occ`` set only reports how many of its
 // workgroups one CU holds at once.
 template <int NT, int MAXR, bool BA, bool IO>
 static hipError_t bp_launch_io(const rf_bp_params& p, hipStream_t stream, int* occ) {
-    const size_t shmem = (size_t)p.nslots * 8u + (p.tab_off ? 4u * (size_t)bp_tab_words(p) : 0u);
+    const size_t shmem = bp_shmem(p);
     const void* fn = reinterpret_cast<const void*>(&rf_bp_kernel<NT, MAXR, BA, IO>);
     static bool attr_set = false;
     if (!attr_set) {

@@ -98,6 +98,7 @@ struct rf_model {
     rf_stats stats{};
     rf_args base{};
     uint32_t* bp_prog = nullptr;   // device copy of the bit-plane program
+    unsigned long long* bp_progress = nullptr;  // progress counter of a bit-plane launch
     rf_bp_params bp{};
     // launches of rf_run_cycles not yet waited for: their HIP events are
     // folded into the statistics (and device flags checked) by the next call
@@ -291,6 +292,8 @@ int load_bitplane(rf_model* m) {
     RF_HIP(hipMalloc(&m->bp_prog, bytes));
     RF_HIP(hipMemcpy(m->bp_prog, b, bytes, hipMemcpyHostToDevice));
     m->bp.prog = m->bp_prog;
+    RF_HIP(hipMalloc(&m->bp_progress, sizeof(unsigned long long)));
+    m->bp.progress = m->bp_progress;
     return RF_OK;
 }
 
@@ -648,7 +651,7 @@ int rf_destroy(rf_model* m) {
     for (int c = 0; c < 4; ++c)
         if (m->cls[c]) (void)hipFree(m->cls[c]);
     void* ps[] = {m->mem, m->change, m->done, m->hash, m->cyc, m->stim, m->flags,
-                  m->budget, m->order, m->out_hash, m->images, m->bp_prog};
+                  m->budget, m->order, m->out_hash, m->images, m->bp_prog, m->bp_progress};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     for (void* p : m->image_bufs)
@@ -927,6 +930,7 @@ int rf_run_cycles(rf_model* m, uint32_t n, uint32_t mode) {
             }
             const std::vector<bp_window> plan = bp_rotation(G, S, chunk, m->cfg.bitplane_slots != 0);
             if (plan.empty()) {
+                RF_HIP(hipMemsetAsync(m->bp_progress, 0, sizeof(unsigned long long), m->stream));
                 RF_HIP(rf_bp_launch(p, m->cfg.bitplane_threads, m->stream));
             } else {
                 p.grp_count = G;
@@ -938,6 +942,7 @@ int rf_run_cycles(rf_model* m, uint32_t n, uint32_t mode) {
                     p.ncyc_hi = w.c_hi;
                     p.fwb_lo = w.f_lo;
                     p.fwb_hi = w.f_hi;
+                    RF_HIP(hipMemsetAsync(m->bp_progress, 0, sizeof(unsigned long long), m->stream));
                     RF_HIP(rf_bp_launch(p, m->cfg.bitplane_threads, m->stream));
                 }
                 launches = (uint32_t)plan.size();
@@ -1364,6 +1369,7 @@ static int bitplane_rounds(rf_model* m, uint64_t seed, const uint32_t* budgets, 
         p.a.budget = m->budget;
         p.a.out_hash = m->out_hash;
         p.a.nstim = nstim;
+        RF_HIP(hipMemsetAsync(m->bp_progress, 0, sizeof(unsigned long long), m->stream));
         RF_HIP(hipEventRecord(m->ev0, m->stream));
         RF_HIP(rf_bp_launch(p, m->cfg.bitplane_threads, m->stream));
         RF_HIP(hipEventRecord(m->ev1, m->stream));
