@@ -46,7 +46,7 @@ exports (``rf_design.bitplane``); its layout is documented on ``encode``.
 """
 from __future__ import annotations
 
-from typing import Dict, List, Optional, Sequence, Tuple
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 from .ir import (Assign, Case, Concat, Cond, Const, Expr, If, Lhs, MemRd, Op, Ref, Sel, Var)
 
@@ -693,11 +693,13 @@ def build_phase(net: Net, roots: Sequence[Lit], fixed: Dict[int, int], nfixed: i
         for x in by_level[L]:
             slot[x] = 0            # destination chosen below
             pending.append((x, _gate(net, x, slot)))
-        ordered = bank_order(pending)
+        ordered = bank_order(pending, direct)
         gates = []
         for r0 in range(0, len(ordered), BANK_GROUP):
-            avoid: set = set()
-            for x, g in ordered[r0:r0 + BANK_GROUP]:
+            run = ordered[r0:r0 + BANK_GROUP]
+            # the run's in-place destinations are fixed: the others avoid them
+            avoid: set = {direct[x] % BANK_MOD for x, _ in run if x in direct}
+            for x, g in run:
                 if x in direct:
                     d = direct[x]
                 else:
@@ -721,30 +723,31 @@ BANK_GROUP = 32
 BANK_MOD = 32
 
 
-def bank_order(pending: List[Tuple[int, Tuple[int, int, int, int, int, int]]]
-               ) -> List[Tuple[int, Tuple[int, int, int, int, int, int]]]:
-    """Order one level's (node, gate) pairs so that each run of BANK_GROUP
-    consecutive gates (the lanes one LDS access serves) spreads its operand
-    reads (s, x, y) over the banks: a run takes a gate only while, per
-    operand, the slot's residue is held by fewer than ``cap`` other slots of
-    the run (equal slots are a broadcast and free); ``cap`` starts at 1 and
-    rises only when no remaining gate fits. Destinations are chosen
-    afterwards, distinct per run. Greedy, level by level."""
-    rest = sorted(pending, key=lambda p: (p[1][1], p[1][2]))
+def bank_runs(items: List, keys: Callable[[object], Tuple[Optional[int], ...]]) -> List:
+    """Order ``items`` so that each run of BANK_GROUP consecutive items (the
+    lanes one LDS access serves) spreads every access over the banks:
+    ``keys(item)`` gives the item's slot per access (None: no constraint); a
+    run takes an item only while, per access, the slot's residue is held by
+    fewer than ``cap`` other slots of the run (equal slots are a broadcast
+    and free); ``cap`` starts at 1 and rises only when no remaining item
+    fits. Greedy."""
+    rest = list(items)
     out = []
     while rest:
-        used: List[Dict[int, set]] = [{}, {}, {}]
+        used: Dict[int, Dict[int, set]] = {}
         run: List[int] = []
         taken = set()
         cap = 1
         while len(run) < min(BANK_GROUP, len(rest)):
             hit = -1
-            for k, (_, g) in enumerate(rest):
+            for k, it in enumerate(rest):
                 if k in taken:
                     continue
                 ok = True
-                for jj, o in enumerate((g[1], g[2], g[4])):
-                    held = used[jj].get(o % BANK_MOD)
+                for jj, o in enumerate(keys(it)):
+                    if o is None:
+                        continue
+                    held = used.get(jj, {}).get(o % BANK_MOD)
                     if held and o not in held and len(held) >= cap:
                         ok = False
                         break
@@ -754,14 +757,26 @@ def bank_order(pending: List[Tuple[int, Tuple[int, int, int, int, int, int]]]
             if hit < 0:
                 cap += 1
                 continue
-            g = rest[hit][1]
-            for jj, o in enumerate((g[1], g[2], g[4])):
-                used[jj].setdefault(o % BANK_MOD, set()).add(o)
+            for jj, o in enumerate(keys(rest[hit])):
+                if o is not None:
+                    used.setdefault(jj, {}).setdefault(o % BANK_MOD, set()).add(o)
             run.append(hit)
             taken.add(hit)
         out += [rest[k] for k in sorted(run)]
         rest = [p for k, p in enumerate(rest) if k not in taken]
     return out
+
+
+def bank_order(pending: List[Tuple[int, Tuple[int, int, int, int, int, int]]],
+               fixed_dst: Optional[Dict[int, int]] = None
+               ) -> List[Tuple[int, Tuple[int, int, int, int, int, int]]]:
+    """Order one level's (node, gate) pairs by ``bank_runs`` over the operand
+    reads (s, x, y) and, for gates whose destination is fixed (``fixed_dst``:
+    next-state values written in place), the write. The other destinations
+    are chosen afterwards, distinct per run. Level by level."""
+    fd = fixed_dst or {}
+    rest = sorted(pending, key=lambda p: (p[1][1], p[1][2]))
+    return bank_runs(rest, lambda p: (p[1][1], p[1][2], p[1][4], fd.get(p[0])))
 
 
 def _lit_slot(lit: Lit, slot: Dict[int, int]) -> Tuple[int, int]:
@@ -984,7 +999,10 @@ def compile_program(comp) -> Program:
     prog.in_chunks = in_chunks
     prog.state_load = state_load
     prog.phases = [phA, phM, phB]
-    prog.commits = [commit_of(phA, slotA), commit_of(phM, slotM)]
+    # commit pairs run in parallel (a pair's next-state slot is its own, never
+    # a state slot): ordered so each run of BANK_GROUP spreads its reads and
+    # writes over the banks
+    prog.commits = [bank_runs(commit_of(ph, sl), lambda c: c) for ph, sl in ((phA, slotA), (phM, slotM))]
     prog.hashes = [hash_of(slotM), hash_of(slotB)]
     prog.mems = mem_table
     prog.ports = [ports_of(slotA), ports_of(slotM)]
