@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define RF_DESIGN_ABI 7
+#define RF_DESIGN_ABI 8
 
 /* storage classes (include/rf_heavy.h:11-17: CData/SData/IData/QData) */
 enum { RF_CLS_C = 0, RF_CLS_S = 1, RF_CLS_I = 2, RF_CLS_Q = 3 };
@@ -151,6 +151,13 @@ typedef struct rf_design {
      * per 64-bit word (NULL when the design is outside that subset) */
     const uint32_t* bitplane;
     uint64_t bitplane_words;
+    /* multi-wavefront fused kernel: one workgroup of 64 x mw_waves threads
+     * per 64 stimuli, each wavefront running its share of every level of the
+     * mtask DAG (NULL / 0 when the design has none); the runtime launches it
+     * for rf_run_cycles in RF_MODE_FUSED when rf_config.fused_multiwave is
+     * set and no input stream or output capture is */
+    const void* k_fused_mw;
+    uint32_t mw_waves;
 } rf_design;
 
 typedef const rf_design* (*rf_design_get_fn)(void);

@@ -53,6 +53,12 @@ typedef struct rf_config {
                                  faster than a launch of several rounds;
                                  N > 0 = rotate through windows of at most N
                                  groups whenever there are more than N */
+    uint32_t fused_multiwave; /* RF_MODE_FUSED in rf_run_cycles: 1 = the
+                                 design's multi-wavefront kernel (one
+                                 workgroup of 64 x rf_design.mw_waves threads
+                                 per 64 stimuli) when it has one and no
+                                 input stream or output capture is set;
+                                 0 = one wavefront per 64 stimuli */
 } rf_config;
 
 typedef struct rf_model rf_model;

@@ -221,6 +221,7 @@ class Compiled:
             "threads": self.opts.threads,
             "block": self.opts.block,
             "form": "calls" if self.calls else "inline",
+            "mw_waves": mw_waves(self),
             "bitplane": self.bitplane_info,
             "streams": self.stream_table(),
         }
@@ -1238,6 +1239,157 @@ def emit_fused(comp: Compiled, io: bool = False) -> str:
     return "\n".join(lines)
 
 
+# ---------------------------------------------------------------------------
+# Multi-wavefront fused kernel: one workgroup serves 64 stimuli with W
+# wavefronts. An eval runs the mtask DAG level by level (a level: the mtasks
+# whose longest predecessor chain has the same length); each wavefront runs
+# its share of a level's mtasks for the 64 lanes, values pass between
+# mtasks through the lane-minor arrays (write-through stores), and a barrier
+# closes every level. Wavefront 0 applies the inputs, sets the clock, runs
+# the clock latch and hashes the outputs. This is the graph form's per-mtask
+# code and dependence structure with the cycle loop inside one kernel, for
+# designs whose per-stimulus logic is wider than one wavefront's issue.
+# ---------------------------------------------------------------------------
+
+MW_MAX_WAVES = 16
+
+
+def mw_plan(comp: Compiled) -> Tuple[int, List[List[List[int]]]]:
+    """(W, levels): the wavefront count (total mtask cost over the costliest
+    dependence chain, at most MW_MAX_WAVES and the widest level) and per
+    level per wavefront the mtask indices, assigned costliest first to the
+    least loaded wavefront."""
+    mts = comp.mtasks
+    idx = {mt.id: k for k, mt in enumerate(mts)}
+    lev: Dict[int, int] = {}
+    cp: Dict[int, int] = {}
+
+    def walk(k: int):
+        if k not in lev:
+            ps = [idx[p] for p in mts[k].preds if p in idx]
+            for q in ps:
+                walk(q)
+            lev[k] = 1 + max((lev[q] for q in ps), default=-1)
+            cp[k] = max(1, mts[k].cost) + max((cp[q] for q in ps), default=0)
+    for k in range(len(mts)):
+        walk(k)
+    nl = 1 + max(lev.values(), default=-1)
+    by_level = [[k for k in range(len(mts)) if lev[k] == L] for L in range(nl)]
+    total = sum(max(1, mt.cost) for mt in mts)
+    crit = max(cp.values(), default=1)
+    W = max(1, min(MW_MAX_WAVES, -(-total // crit), max((len(b) for b in by_level), default=1)))
+    out = []
+    for ks in by_level:
+        load = [0] * W
+        waves: List[List[int]] = [[] for _ in range(W)]
+        for k in sorted(ks, key=lambda k: -max(1, mts[k].cost)):
+            w = min(range(W), key=lambda i: load[i])
+            waves[w].append(k)
+            load[w] += max(1, mts[k].cost)
+        out.append([sorted(x) for x in waves])
+    return W, out
+
+
+def mw_waves(comp: Compiled) -> int:
+    """Wavefronts of the design's rf_k_fused_mw, 0 when it has none."""
+    if comp.host or comp.calls or not comp.mtasks or comp.sch.change_vars:
+        return 0
+    W = mw_plan(comp)[0]
+    return W if W >= 2 else 0
+
+
+def _mw_chunk(gen: Gen, ind: str) -> List[str]:
+    """A generated code fragment as one scope."""
+    return [ind + "{"] + [ind + "    " + ln.strip() for ln in gen.top_lines + gen.lines] + [ind + "}"]
+
+
+def emit_fused_mw(comp: Compiled) -> Tuple[Optional[str], int]:
+    """(source of rf_k_fused_mw, W), or (None, 0) where the form does not
+    apply: the host build, the calls form, designs with change-detected
+    variables (their settle loop stays with the one-wavefront forms) and
+    designs with too little parallel logic for two wavefronts."""
+    if not mw_waves(comp):
+        return None, 0
+    W, levels = mw_plan(comp)
+    d = comp.design
+    L = [f'extern "C" __global__ void __launch_bounds__({64 * W}) rf_k_fused_mw(rf_args a) {{',
+         "    const uint32_t wave = threadIdx.x >> 6;",
+         "    const uint32_t lane = blockIdx.x * 64u + (threadIdx.x & 63u);",
+         "    const bool live = lane < a.nlanes && a.done[lane] == 0u;",
+         "    size_t NP = a.npad;",
+         "    uint8_t* __restrict__ sC = a.c + lane;",
+         "    uint16_t* __restrict__ sS = a.s + lane;",
+         "    uint32_t* __restrict__ sI = a.i + lane;",
+         "    uint64_t* __restrict__ sQ = a.q + lane;"]
+    for m, off in comp.layout.lane_major.items():
+        c = class_of(m)
+        row = m.depth * (CLASS_BYTES[m.cls] if m.cls != "W" else 4 * m.words)
+        L.append(f"    {CTY[c]}* __restrict__ mb{m.id} = ({CTY[c]}*)(a.mem + (size_t){off} * NP"
+                 f" + (size_t)lane * {row}u);")
+    L += ["    (void)sC; (void)sS; (void)sI; (void)sQ; (void)NP;",
+          "    uint64_t h = 0ull, sid = 0ull;",
+          "    uint32_t cyc = 0u;",
+          "    if (live) {",
+          "        h = a.hash[lane];",
+          "        cyc = a.cyc[lane];",
+          "        sid = (uint64_t)a.stim[lane];",
+          "    }",
+          "    for (uint32_t n = 0u; n < a.ncycles; ++n, ++cyc) {"]
+
+    def wgen(local) -> Gen:
+        g = Gen(comp)
+        g.top_lines = []
+        g.write_through = True
+        g.local = set(local)
+        return g
+
+    g = wgen(d.inputs)
+    _stim_code(comp, g, "sid", "(uint64_t)cyc", False, False)
+    for v in sorted(g.dirty, key=lambda x: x.id):
+        g.store(v)
+    L.append("        if (wave == 0u && live)")
+    L += _mw_chunk(g, "        ")
+    L += ["        #pragma unroll 1",
+          "        for (uint32_t ph = 0u; ph < 2u; ++ph) {"]
+    g = wgen(comp.in_clocks)
+    for clk in comp.in_clocks:
+        g.declare(clk)
+        g.emit(g.set(clk, "ph"))
+        g.store(clk)
+    L.append("            if (wave == 0u && live)")
+    L += _mw_chunk(g, "            ")
+    L.append("            __syncthreads();")
+    for lv in levels:
+        busy = [(w, ks) for w, ks in enumerate(lv) if ks]
+        L.append("            if (live) {")
+        for i, (w, ks) in enumerate(busy):
+            L.append(f"                {'if' if i == 0 else '} else if'} (wave == {w}u) {{")
+            for k in ks:
+                L += _mw_chunk(_mtask_gen(comp, k), "                    ")
+        L += ["                }", "            }", "            __syncthreads();"]
+    g = wgen(set(d.clocks) | set(comp.clklast.values()))
+    for clk, last in comp.clklast.items():
+        g.ensure(clk)
+        g.declare(last)
+        g.emit(g.set(last, g.get(clk)))
+        g.avail.add(last)
+        g.store(last)
+    L.append("            if (wave == 0u && live)")
+    L += _mw_chunk(g, "            ")
+    L += ["            __syncthreads();", "        }"]
+    g = wgen(d.outputs)
+    _hash_code(comp, g, "h")
+    L.append("        if (wave == 0u && live)")
+    L += _mw_chunk(g, "        ")
+    L += ["    }",
+          "    if (wave == 0u && live) {",
+          "        a.hash[lane] = h;",
+          "        a.cyc[lane] = cyc;",
+          "    }",
+          "}"]
+    return "\n".join(L), W
+
+
 def _opaque_bases(gen: Gen):
     """Before a kernel's final stores: the slot base pointers and stride
     through an optimisation barrier (RF_OPAQUE_*), so the stores recompute
@@ -1317,6 +1469,10 @@ def _emit_source(comp: Compiled) -> str:
             parts.append("")
             parts.append(emit_persistent(comp, io))
             parts.append("")
+        mw, _ = emit_fused_mw(comp)
+        if mw:
+            parts.append(mw)
+            parts.append("")
     tail = emit_settle(comp)
     if tail:
         parts.append(tail)
@@ -1394,9 +1550,11 @@ def emit_descriptor(comp: Compiled, h: int) -> str:
     out.append(f"    {man['bytes_seq']}ull, 0ull, {man['bytes_in']}ull, {man['bytes_out']}ull,")
     out.append(f"    {man['ops_per_cycle']}ull,")
     if comp.bitplane_blob:
-        out.append(f"    rf_bitplane, {len(comp.bitplane_blob)}ull")
+        out.append(f"    rf_bitplane, {len(comp.bitplane_blob)}ull,")
     else:
-        out.append("    0, 0ull")
+        out.append("    0, 0ull,")
+    mw_w = mw_waves(comp)
+    out.append(f"    (const void*)&rf_k_fused_mw, {mw_w}u" if mw_w else "    0, 0u")
     out.append("};")
     # the only exported symbol: kernels stay library-local so several design
     # libraries can live in one process

@@ -546,6 +546,7 @@ int rf_create(const rf_design* d, uint32_t n_lanes, const rf_config* cfg, rf_mod
         if (cfg->bitplane_threads) m->cfg.bitplane_threads = cfg->bitplane_threads;
         m->cfg.lane_pad = cfg->lane_pad;
         m->cfg.bitplane_slots = cfg->bitplane_slots;
+        m->cfg.fused_multiwave = cfg->fused_multiwave;
     }
     m->nlanes = n_lanes;
     const uint64_t q = d->block > 64 ? d->block : 64;
@@ -910,8 +911,16 @@ int rf_run_cycles(rf_model* m, uint32_t n, uint32_t mode) {
         if (mode == RF_MODE_FUSED) {
             rf_args a = args_for(m, RF_PHASE_EVAL);
             a.ncycles = chunk;
-            rc = launch(m, streams_set(m) ? m->d->k_fused_io : m->d->k_fused, a);
-            if (rc) return rc;
+            if (m->cfg.fused_multiwave && m->d->k_fused_mw && m->d->mw_waves >= 2 && m->d->mw_waves <= 16 &&
+                !streams_set(m)) {
+                // one workgroup of mw_waves wavefronts per 64 stimuli
+                void* params[] = {&a};
+                RF_HIP(hipLaunchKernel(m->d->k_fused_mw, dim3((m->nlanes + 63u) / 64u), dim3(64u * m->d->mw_waves),
+                                       params, 0, m->stream));
+            } else {
+                rc = launch(m, streams_set(m) ? m->d->k_fused_io : m->d->k_fused, a);
+                if (rc) return rc;
+            }
         }
         uint32_t launches = 1;
         if (mode == RF_MODE_BITPLANE) {

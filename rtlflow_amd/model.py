@@ -33,7 +33,8 @@ class RFError(RuntimeError):
 class rf_config(C.Structure):
     _fields_ = [("settle_unroll", C.c_uint32), ("converge_limit", C.c_uint32),
                 ("fused_chunk", C.c_uint32), ("bitplane_threads", C.c_uint32),
-                ("lane_pad", C.c_uint32), ("bitplane_slots", C.c_uint32)]
+                ("lane_pad", C.c_uint32), ("bitplane_slots", C.c_uint32),
+                ("fused_multiwave", C.c_uint32)]
 
 
 class rf_stats(C.Structure):
@@ -151,12 +152,12 @@ class Model:
 
     def __init__(self, design: DesignLib, n_lanes: int, settle_unroll: int = 2,
                  converge_limit: int = 100, fused_chunk: int = 512, bitplane_threads: int = 0,
-                 lane_pad: int = 0, bitplane_slots: int = 0):
+                 lane_pad: int = 0, bitplane_slots: int = 0, fused_multiwave: int = 0):
         self.rt = runtime()
         self.design = design
         self.n = int(n_lanes)
         cfg = rf_config(settle_unroll, converge_limit, fused_chunk, bitplane_threads, lane_pad,
-                        bitplane_slots)
+                        bitplane_slots, fused_multiwave)
         h = C.c_void_p()
         _check(self.rt.rf_create(design.desc, self.n, C.byref(cfg), C.byref(h)))
         self.h = h
