@@ -99,6 +99,7 @@ struct rf_model {
     rf_args base{};
     uint32_t* bp_prog = nullptr;   // device copy of the bit-plane program
     unsigned long long* bp_progress = nullptr;  // progress counter of a bit-plane launch
+    int bp_per_cu[2] = {0, 0};     // bit-plane workgroups per CU (plain / stream kernel), 0 = not asked yet
     rf_bp_params bp{};
     // launches of rf_run_cycles not yet waited for: their HIP events are
     // folded into the statistics (and device flags checked) by the next call
@@ -933,9 +934,12 @@ int rf_run_cycles(rf_model* m, uint32_t n, uint32_t mode) {
             const uint32_t G = (m->nlanes + 63u) / 64u;
             uint32_t S = m->cfg.bitplane_slots;
             if (!S) {
-                int per_cu = 0;
-                RF_HIP(rf_bp_occupancy(p, m->cfg.bitplane_threads, &per_cu));
-                S = (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, m->ncu);
+                int& per_cu = m->bp_per_cu[(p.a.in.data || p.a.out.data) ? 1 : 0];
+                if (!per_cu) {
+                    RF_HIP(rf_bp_occupancy(p, m->cfg.bitplane_threads, &per_cu));
+                    per_cu = std::max(1, per_cu);
+                }
+                S = (uint32_t)per_cu * (uint32_t)std::max(1, m->ncu);
             }
             const std::vector<bp_window> plan = bp_rotation(G, S, chunk, m->cfg.bitplane_slots != 0);
             if (plan.empty()) {
