@@ -99,3 +99,28 @@ def test_rotation_with_input_stream_and_capture(gpu):
                              inputs=np.ascontiguousarray(words[:, :, k0:k0 + k]), capture=want)
     assert [int(v) for v in hs[0][k0:k0 + k]] == ref
     assert (caps[0][:, :, k0:k0 + k] == want).all()
+
+
+@pytest.mark.gpu
+def test_rotation_at_the_headline_shape(gpu):
+    """net2 at 65,536 stimuli (1,024 groups against 768 workgroup slots at
+    three per CU): the automatic rotation and one launch over all groups
+    (window wider than the groups) give the same hash for every stimulus and
+    the same lane state, and a strided sample matches the oracle."""
+    cd = registry.get("net2")
+    lanes, cycles = 65536, 300
+    print(f"rotation net2 lanes={lanes} cycles={cycles}", flush=True)
+    with cd.model(lanes, fused_chunk=cycles) as a, cd.model(lanes, fused_chunk=cycles, bitplane_slots=4096) as b:
+        for m in (a, b):
+            m.begin(SEED)
+            m.run_cycles(cycles, MODE_BITPLANE)
+        got = a.hashes()
+        assert a.stats()["bitplane_launches"] > 1
+        assert b.stats()["bitplane_launches"] == 1
+        assert (got == b.hashes()).all()
+        for lane in (0, 767 * 64 + 5, 768 * 64, lanes - 1):
+            assert a.lane_state(lane) == b.lane_state(lane), lane
+    tb = registry.table_bytes("net2")
+    for g in (0, 511, 767, 768, 1023):
+        want = oracle.netlist_run(tb, SEED, 64, cycles, stim0=g * 64)
+        assert (got[g * 64:(g + 1) * 64] == want).all(), f"group {g}"
