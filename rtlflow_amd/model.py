@@ -321,8 +321,10 @@ class Model:
             raise RFError(f"{what}: expected a contiguous tensor of 64-bit words")
         # the model's stream does not wait for torch's: finish the work that
         # fills (or zeroes) the buffer before the model's kernels use it
-        import torch
-        torch.cuda.current_stream(buf.device).synchronize()
+        # (a buffer in host memory is the runtime's to refuse)
+        if buf.is_cuda:
+            import torch
+            torch.cuda.current_stream(buf.device).synchronize()
         return buf.data_ptr(), shape[0], shape[2]
 
     def set_input_stream(self, buf, stim0: int = 0, cycle0: int = 0):
