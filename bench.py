@@ -503,18 +503,20 @@ def bench_other(args):
         nchk = min(args.check, lanes)
         ref = np.concatenate([run(seed, nchk, cycles, threads=th),
                               run(seed, nchk, cycles, stim0=lanes - nchk, threads=th)])
-        runs = [("fused", cd, MODE_FUSED, args.chunk, 0), ("graph", cd, MODE_GRAPH, args.chunk, 0)]
+        runs = [("fused", cd, MODE_FUSED, args.chunk, 0, 0), ("graph", cd, MODE_GRAPH, args.chunk, 0, 0)]
         if f"{wl}_tuned" in registry.load_index():
             # the autotuner's choice of compile-time options and launch shape
+            # (with the multi-wavefront fused kernel when it chose that)
             from rtlflow_amd.autotune import choice
             t = choice(wl)
             runs.append(("tuned", registry.get(f"{wl}_tuned"), MODE_FUSED,
-                         t["knobs"].get("chunk", args.chunk), t["knobs"].get("lane_pad", 0)))
+                         t["knobs"].get("chunk", args.chunk), t["knobs"].get("lane_pad", 0),
+                         t["knobs"].get("multiwave", 0)))
         if args.main_only:
             runs = runs[-1:]
-        for form, c, mode, chunk, pad in runs:
-            log(f"[bench] {wl} form={form} lanes={lanes} cycles={cycles}")
-            with c.model(lanes, fused_chunk=chunk, lane_pad=pad) as m:
+        for form, c, mode, chunk, pad, mw in runs:
+            log(f"[bench] {wl} form={form} lanes={lanes} cycles={cycles} multiwave={mw}")
+            with c.model(lanes, fused_chunk=chunk, lane_pad=pad, fused_multiwave=mw) as m:
                 m.begin(seed)
                 m.run_cycles(0, mode)
                 m.sync()
@@ -530,11 +532,13 @@ def bench_other(args):
             if mode == MODE_FUSED:
                 per = st["fused_ms"] / max(1, st["fused_launches"])
                 # the committed counters are the reported (last) form's
-                rl = roofline_line(b_alg, lanes * min(chunk, cycles), per, "rf_k_fused",
-                                   f"{wl}_{form}_traffic.json",
-                                   {"lanes": lanes, "cycles_per_launch": min(chunk, cycles),
-                                    "block": c.manifest["block"]},
-                                   f"{wl}_{form}_sq.json")
+                kname = "rf_k_fused_mw" if mw else "rf_k_fused"
+                shape = {"kernel": kname, "lanes": lanes, "cycles_per_launch": min(chunk, cycles),
+                         "block": 64 * c.manifest["mw_waves"] if mw else c.manifest["block"]}
+                # stimulus-cycles of the average launch (the last one runs
+                # what is left of the workload)
+                rl = roofline_line(b_alg, lanes * cycles / max(1, st["fused_launches"]), per, kname,
+                                   f"{wl}_{form}_traffic.json", shape, f"{wl}_{form}_sq.json")
             else:
                 per = st["graph_ms"] / max(1, st["graph_launches"])
                 rl = roofline_line(b_alg, lanes, per, "cycle graph")

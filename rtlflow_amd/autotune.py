@@ -15,6 +15,9 @@ Knobs:
   - the form (bit-plane, fused, plain graph, batched queue);
   - bit-plane workgroup size (256 / 512 / 1,024 threads per 64 lanes);
   - fused launch length (cycles per launch);
+  - the fused form's wavefronts per 64 stimuli: one, or the design's
+    multi-wavefront kernel (rf_config.fused_multiwave, the mtask levels
+    spread over several wavefronts with the lanes' slots in LDS);
   - batched form: lanes in the queue;
   - the SoA padding (extra lanes in every slot's row, rf_config.lane_pad)
     of the per-lane forms.
@@ -137,6 +140,8 @@ def _measure(cd, name: str, wl: Dict, form: str, knobs: Dict):
         kw["fused_chunk"] = knobs["chunk"]
     if "lane_pad" in knobs:
         kw["lane_pad"] = knobs["lane_pad"]
+    if knobs.get("multiwave"):
+        kw["fused_multiwave"] = 1
     if wl["kind"] == "queue":
         lanes = knobs["lanes"]
         with cd.model(lanes, **kw) as m:
@@ -176,6 +181,8 @@ def runtime_candidates(name: str, man: Dict, wl: Dict) -> List[tuple]:
     for chunk in (64, 512):
         for pad in LANE_PAD:
             out.append(("fused", {"chunk": chunk, "lane_pad": pad}))
+    if man.get("mw_waves", 0) >= 2:
+        out.append(("fused", {"chunk": 512, "multiwave": 1}))
     out.append(("graph", {}))
     return out
 

@@ -1298,6 +1298,19 @@ def mw_waves(comp: Compiled) -> int:
     return W if W >= 2 else 0
 
 
+# LDS of one multi-wavefront workgroup that may hold its lanes' slots
+MW_LDS_MAX = 150 * 1024
+
+
+def mw_lds_slots(comp: Compiled) -> Optional[Dict[str, int]]:
+    """Slots per class of the design when the 64 lanes' worth of them fits
+    the multi-wavefront kernel's LDS (the kernel then keeps them there for a
+    launch), else None (slots stay in the lane-minor arrays)."""
+    sl = comp.layout.slots
+    total = sum(sl[c] * CLASS_BYTES[c] for c in ("C", "S", "I", "Q")) * 64
+    return dict(sl) if total <= MW_LDS_MAX else None
+
+
 def _mw_chunk(gen: Gen, ind: str) -> List[str]:
     """A generated code fragment as one scope."""
     return [ind + "{"] + [ind + "    " + ln.strip() for ln in gen.top_lines + gen.lines] + [ind + "}"]
@@ -1312,19 +1325,42 @@ def emit_fused_mw(comp: Compiled) -> Tuple[Optional[str], int]:
         return None, 0
     W, levels = mw_plan(comp)
     d = comp.design
+    sl = mw_lds_slots(comp)
     L = [f'extern "C" __global__ void __launch_bounds__({64 * W}) rf_k_fused_mw(rf_args a) {{',
          "    const uint32_t wave = threadIdx.x >> 6;",
-         "    const uint32_t lane = blockIdx.x * 64u + (threadIdx.x & 63u);",
-         "    const bool live = lane < a.nlanes && a.done[lane] == 0u;",
-         "    size_t NP = a.npad;",
-         "    uint8_t* __restrict__ sC = a.c + lane;",
-         "    uint16_t* __restrict__ sS = a.s + lane;",
-         "    uint32_t* __restrict__ sI = a.i + lane;",
-         "    uint64_t* __restrict__ sQ = a.q + lane;"]
+         "    const uint32_t t64 = threadIdx.x & 63u;",
+         "    const uint32_t lane = blockIdx.x * 64u + t64;",
+         "    const bool live = lane < a.nlanes && a.done[lane] == 0u;"]
+    if sl is None:
+        L += ["    size_t NP = a.npad;",
+              "    uint8_t* __restrict__ sC = a.c + lane;",
+              "    uint16_t* __restrict__ sS = a.s + lane;",
+              "    uint32_t* __restrict__ sI = a.i + lane;",
+              "    uint64_t* __restrict__ sQ = a.q + lane;"]
+    else:
+        # the workgroup's 64 lanes of every slot in LDS, [slot][lane]: the
+        # mtask bodies' slot accesses become LDS accesses (stride 64), and
+        # values cross wavefronts there; loaded at the start of a launch and
+        # written back at its end
+        L.append("    constexpr size_t NP = 64;")
+        for c in ("C", "S", "I", "Q"):
+            if sl[c]:
+                L.append(f"    __shared__ {CTY[c]} lsoa{c}[{sl[c] * 64}];")
+                L.append(f"    {CTY[c]}* __restrict__ s{c} = lsoa{c} + t64;")
+            else:
+                L.append(f"    {CTY[c]}* __restrict__ s{c} = nullptr;")
+        L.append("    {")
+        L.append("        const size_t g0 = (size_t)blockIdx.x * 64u;")
+        for c in ("C", "S", "I", "Q"):
+            if sl[c]:
+                L.append(f"        for (uint32_t k = threadIdx.x; k < {sl[c] * 64}u; k += {64 * W}u)")
+                L.append(f"            lsoa{c}[k] = a.{c.lower()}[(size_t)(k >> 6) * a.npad + g0 + (k & 63u)];")
+        L.append("    }")
+        L.append("    __syncthreads();")
     for m, off in comp.layout.lane_major.items():
         c = class_of(m)
         row = m.depth * (CLASS_BYTES[m.cls] if m.cls != "W" else 4 * m.words)
-        L.append(f"    {CTY[c]}* __restrict__ mb{m.id} = ({CTY[c]}*)(a.mem + (size_t){off} * NP"
+        L.append(f"    {CTY[c]}* __restrict__ mb{m.id} = ({CTY[c]}*)(a.mem + (size_t){off} * a.npad"
                  f" + (size_t)lane * {row}u);")
     L += ["    (void)sC; (void)sS; (void)sI; (void)sQ; (void)NP;",
           "    uint64_t h = 0ull, sid = 0ull;",
@@ -1359,14 +1395,36 @@ def emit_fused_mw(comp: Compiled) -> Tuple[Optional[str], int]:
     L.append("            if (wave == 0u && live)")
     L += _mw_chunk(g, "            ")
     L.append("            __syncthreads();")
-    for lv in levels:
-        busy = [(w, ks) for w, ks in enumerate(lv) if ks]
-        L.append("            if (live) {")
-        for i, (w, ks) in enumerate(busy):
-            L.append(f"                {'if' if i == 0 else '} else if'} (wave == {w}u) {{")
-            for k in ks:
-                L += _mw_chunk(_mtask_gen(comp, k), "                    ")
-        L += ["                }", "            }", "            __syncthreads();"]
+
+    def level_lines(lvs, ind):
+        out = []
+        for lv in lvs:
+            busy = [(w, ks) for w, ks in enumerate(lv) if ks]
+            if not busy:
+                continue
+            out.append(ind + "if (live) {")
+            for i, (w, ks) in enumerate(busy):
+                out.append(ind + f"    {'if' if i == 0 else '} else if'} (wave == {w}u) {{")
+                for k in ks:
+                    out.extend(_mw_chunk(_mtask_gen(comp, k), ind + "        "))
+            out += [ind + "    }", ind + "}", ind + "__syncthreads();"]
+        return out
+
+    cone = clk0_cone(comp)
+    if cone is None:
+        L += level_lines(levels, "            ")
+    else:
+        # after the first cycle of a launch, the clk = 0 eval runs only the
+        # mtasks holding falling-edge cone logic (clk0_cone()): every other
+        # value still holds in the slots since the last settle
+        inside = set(cone)
+        in_cone = [any(v in inside for v in mt.vids) for mt in comp.mtasks]
+        cone_levels = [[[k for k in ks if in_cone[k]] for ks in lv] for lv in levels]
+        L.append("            if (ph == 0u && n != 0u) {")
+        L += level_lines(cone_levels, "                ")
+        L.append("            } else {")
+        L += level_lines(levels, "                ")
+        L.append("            }")
     g = wgen(set(d.clocks) | set(comp.clklast.values()))
     for clk, last in comp.clklast.items():
         g.ensure(clk)
@@ -1385,8 +1443,17 @@ def emit_fused_mw(comp: Compiled) -> Tuple[Optional[str], int]:
           "    if (wave == 0u && live) {",
           "        a.hash[lane] = h;",
           "        a.cyc[lane] = cyc;",
-          "    }",
-          "}"]
+          "    }"]
+    if sl is not None:
+        L += ["    __syncthreads();",
+              "    {",
+              "        const size_t g0 = (size_t)blockIdx.x * 64u;"]
+        for c in ("C", "S", "I", "Q"):
+            if sl[c]:
+                L.append(f"        for (uint32_t k = threadIdx.x; k < {sl[c] * 64}u; k += {64 * W}u)")
+                L.append(f"            a.{c.lower()}[(size_t)(k >> 6) * a.npad + g0 + (k & 63u)] = lsoa{c}[k];")
+        L.append("    }")
+    L.append("}")
     return "\n".join(L), W
 
 

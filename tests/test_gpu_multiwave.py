@@ -75,3 +75,26 @@ def test_multiwave_done_lanes_and_handover(gpu):
         assert (a.hashes() == b.hashes()).all()
         for lane in (0, 5, 6, 64, 199):
             assert a.lane_state(lane) == b.lane_state(lane), lane
+
+
+@pytest.mark.gpu
+def test_tuned_choice_matches_oracle(gpu):
+    """The autotuner's choice for the systolic array (its library, launch
+    length and, when chosen, the multi-wavefront kernel) against the oracle
+    over a ragged lane count and runs split across launches."""
+    from rtlflow_amd.autotune import choice
+    if "systolic_tuned" not in registry.load_index():
+        pytest.skip("systolic_tuned not built")
+    t = choice("systolic")
+    k = t["knobs"]
+    cd = registry.get("systolic_tuned")
+    n, seed = 520, 0x5C0
+    print(f"tuned systolic knobs={k} waves={cd.manifest['mw_waves']}", flush=True)
+    with cd.model(n, fused_chunk=k.get("chunk", 512), lane_pad=k.get("lane_pad", 0),
+                  fused_multiwave=k.get("multiwave", 0)) as m:
+        m.begin(seed)
+        m.run_cycles(33, MODE_FUSED)
+        m.run_cycles(40, MODE_FUSED)
+        got = m.hashes()
+    want = oracle.systolic_run(seed, n, 73)
+    assert (got == want).all(), f"{int((got != want).sum())} of {n} differ"
