@@ -11,6 +11,8 @@ Knobs:
   - ``inline_limit``: all state of a fused kernel in registers, or the
     "calls" form whose mtask bodies work on the lane-minor arrays (for
     designs whose state does not fit in registers);
+  - ``split_reductions``: long combinational reductions split into partial
+    terms that the partitioner can place in separate mtasks;
 * run time:
   - the form (bit-plane, fused, plain graph, batched queue);
   - bit-plane workgroup size (256 / 512 / 1,024 threads per 64 lanes);
@@ -61,6 +63,8 @@ COMPILE_SPACE = {
 }
 # SoA padding: extra lanes in every slot's row (rf_config.lane_pad)
 LANE_PAD = (0, 64)
+# operands per partial term of a split reduction (Options.split_reductions)
+SPLIT_REDUCTIONS = 32
 MEM_LAYOUT = (64, 1 << 30)
 
 
@@ -93,7 +97,23 @@ def compile_candidates(name: str) -> List[Dict]:
     for t in COMPILE_SPACE["threads"]:
         if t != base.get("threads"):
             out.append({"threads": t, "block": 256, "lane_major_min_depth": 64, "inline_limit": 6000})
+    # long combinational reductions split into partial terms, at a fine and
+    # a coarse granularity (only designs that have such a reduction)
+    if _long_reductions(name, SPLIT_REDUCTIONS):
+        for t in (4, 16):
+            out.append({"threads": t, "block": 256, "lane_major_min_depth": 64, "inline_limit": 6000,
+                        "split_reductions": SPLIT_REDUCTIONS})
     return out
+
+
+def _long_reductions(name: str, max_terms: int) -> bool:
+    """Whether the design has a combinational reduction of more than
+    ``max_terms`` operands (what the split_reductions option splits)."""
+    from .designs import registry
+    from .order import split_reductions
+    from .verilog import elaborate
+    spec = registry.ENTRIES[name].make()
+    return split_reductions(elaborate(spec["verilog"], spec["top"]), max_terms) > 0
 
 
 def build_candidates(name: str, jobs: int = 8, log=print) -> Dict[str, str]:

@@ -16,7 +16,7 @@ from .emit_hip import ARR, CTY, Gen, cty, edge_cond, is_wide, lit
 from .instrcount import process_cost
 from .ir import CLASS_BYTES, Assign, Design, Ref, Var, words_of
 from .layout import Layout, class_of
-from .order import CompileError, Schedule, build_schedule, split_seq_processes
+from .order import CompileError, Schedule, build_schedule, split_reductions, split_seq_processes
 from .partition import MTask, partition, partition_stats
 
 BLOCK = 256
@@ -29,7 +29,7 @@ class Options:
                  settle_unroll: int = 2,
                  converge_limit: int = 100, state_in_regs: bool = True, keep=None,
                  mtask_stats: bool = False, inline_limit: int = 6000, split_always: bool = True,
-                 mtask_waves: int = 4, bitplane: bool = True):
+                 mtask_waves: int = 4, bitplane: bool = True, split_reductions: int = 0):
         self.threads = threads            # --threads: partition granularity (cpLimit)
         self.block = block
         self.lane_major_min_depth = lane_major_min_depth
@@ -47,6 +47,9 @@ class Options:
         self.mtask_waves = mtask_waves
         # also lower the design to the bit-plane form when it is in that subset
         self.bitplane = bitplane
+        # split combinational reductions (one associative operator over more
+        # operands than this) into partial terms; 0 = off
+        self.split_reductions = split_reductions
 
     def key(self) -> str:
         return f"t{self.threads}_b{self.block}_lm{self.lane_major_min_depth}"
@@ -61,6 +64,8 @@ class Compiled:
         d = design
         if self.opts.split_always:
             split_seq_processes(d)
+        if self.opts.split_reductions:
+            split_reductions(d, self.opts.split_reductions)
         # synthetic clock-latch variables (__Vclklast__, src/V3Clock.cpp:64-92)
         self.clklast: Dict[Var, Var] = {}
         # the clocks the protocol drives (primary inputs); derived clocks are

@@ -22,7 +22,7 @@ from __future__ import annotations
 
 from typing import Dict, List, Optional, Set, Tuple
 
-from .ir import (Assign, Case, Design, If, Lhs, MemRd, Process, Ref, Stmt, Var, expr_reads,
+from .ir import (Assign, Case, Design, Expr, If, Lhs, MemRd, Op, Process, Ref, Stmt, Var, expr_reads,
                  stmt_exprs, walk_stmts)
 
 
@@ -206,6 +206,76 @@ def split_seq_processes(design: Design) -> int:
         added += len(groups) - 1
     design.processes = out
     design.finalize()
+    return added
+
+
+# associative, commutative operators of equal-width operands (modulo 2^width)
+ASSOC_OPS = ("xor", "or", "and", "add")
+
+
+def _assoc_leaves(e: Expr, op: str, width: int, out: List[Expr]):
+    if isinstance(e, Op) and e.op == op and e.width == width and all(a.width == width for a in e.args):
+        for a in e.args:
+            _assoc_leaves(a, op, width, out)
+    else:
+        out.append(e)
+
+
+def _assoc_chain(terms: List[Expr], op: str, width: int) -> Expr:
+    """A balanced tree of ``op`` over ``terms``."""
+    while len(terms) > 1:
+        nxt = [Op(op, [terms[i], terms[i + 1]], width) for i in range(0, len(terms) - 1, 2)]
+        if len(terms) % 2:
+            nxt.append(terms[-1])
+        terms = nxt
+    return terms[0]
+
+
+def split_reductions(design: Design, max_terms: int) -> int:
+    """Split a combinational assignment whose value is one associative
+    operator (xor, or, and, add) over more than ``max_terms`` operands into
+    partial terms of at most ``max_terms`` operands, each its own
+    combinational process writing a new internal variable, and a final
+    assignment over the partials. Exact: the operators are associative and
+    commutative modulo 2^width. The partials become separate vertices of the
+    eval DAG, so the partitioner and the multi-wavefront plan can spread one
+    long reduction (a 256-term output XOR) over several mtasks. Returns the
+    number of new processes."""
+    if max_terms < 2:
+        return 0
+    out: List[Process] = []
+    added = 0
+    for p in design.processes:
+        st = p.stmts[0] if p.kind == "comb" and len(p.stmts) == 1 else None
+        if not (isinstance(st, Assign) and not st.nb and st.lhs.idx is None and st.lhs.lsb is None
+                and isinstance(st.rhs, Op) and st.rhs.op in ASSOC_OPS):
+            out.append(p)
+            continue
+        w = st.rhs.width
+        leaves: List[Expr] = []
+        _assoc_leaves(st.rhs, st.rhs.op, w, leaves)
+        if len(leaves) <= max_terms:
+            out.append(p)
+            continue
+        # partial terms, level by level, until the final assignment has at
+        # most max_terms operands
+        terms, lvl = leaves, 0
+        while len(terms) > max_terms:
+            parts: List[Expr] = []
+            for k in range(0, len(terms), max_terms):
+                v = Var(f"__Vpart__{st.lhs.var.name}_{lvl}_{k // max_terms}", w, False, "wire")
+                design.vars.append(v)
+                out.append(Process("comb", [Assign(Lhs(v), _assoc_chain(terms[k:k + max_terms], st.rhs.op, w),
+                                                   False)], name=f"{p.name}#part{lvl}.{k // max_terms}",
+                                   line=p.line))
+                parts.append(Ref(v))
+                added += 1
+            terms, lvl = parts, lvl + 1
+        out.append(Process("comb", [Assign(st.lhs, _assoc_chain(terms, st.rhs.op, w), False)],
+                           name=p.name, line=p.line))
+    if added:
+        design.processes = out
+        design.finalize()
     return added
 
 
