@@ -99,10 +99,17 @@ def compile_candidates(name: str) -> List[Dict]:
             out.append({"threads": t, "block": 256, "lane_major_min_depth": 64, "inline_limit": 6000})
     # long combinational reductions split into partial terms, at a fine and
     # a coarse granularity (only designs that have such a reduction)
+    # (with the multi-wavefront kernel's wavefronts capped at 8 for the finer
+    # partitions, whose plans would otherwise take more wavefronts than their
+    # registers allow)
     if _long_reductions(name, SPLIT_REDUCTIONS):
-        for t in (4, 16):
-            out.append({"threads": t, "block": 256, "lane_major_min_depth": 64, "inline_limit": 6000,
-                        "split_reductions": SPLIT_REDUCTIONS})
+        for t, sr, mw in ((4, SPLIT_REDUCTIONS, 16), (16, SPLIT_REDUCTIONS, 16), (8, SPLIT_REDUCTIONS, 8),
+                          (16, SPLIT_REDUCTIONS, 8), (4, SPLIT_REDUCTIONS // 2, 16)):
+            o = {"threads": t, "block": 256, "lane_major_min_depth": 64, "inline_limit": 6000,
+                 "split_reductions": sr}
+            if mw != 16:
+                o["mw_max_waves"] = mw
+            out.append(o)
     return out
 
 

@@ -29,7 +29,8 @@ class Options:
                  settle_unroll: int = 2,
                  converge_limit: int = 100, state_in_regs: bool = True, keep=None,
                  mtask_stats: bool = False, inline_limit: int = 6000, split_always: bool = True,
-                 mtask_waves: int = 4, bitplane: bool = True, split_reductions: int = 0):
+                 mtask_waves: int = 4, bitplane: bool = True, split_reductions: int = 0,
+                 mw_max_waves: int = 16):
         self.threads = threads            # --threads: partition granularity (cpLimit)
         self.block = block
         self.lane_major_min_depth = lane_major_min_depth
@@ -50,6 +51,9 @@ class Options:
         # split combinational reductions (one associative operator over more
         # operands than this) into partial terms; 0 = off
         self.split_reductions = split_reductions
+        # most wavefronts per 64 stimuli of the multi-wavefront kernel (more
+        # wavefronts leave each fewer registers)
+        self.mw_max_waves = mw_max_waves
 
     def key(self) -> str:
         return f"t{self.threads}_b{self.block}_lm{self.lane_major_min_depth}"
@@ -1282,7 +1286,8 @@ def mw_plan(comp: Compiled) -> Tuple[int, List[List[List[int]]]]:
     by_level = [[k for k in range(len(mts)) if lev[k] == L] for L in range(nl)]
     total = sum(max(1, mt.cost) for mt in mts)
     crit = max(cp.values(), default=1)
-    W = max(1, min(MW_MAX_WAVES, -(-total // crit), max((len(b) for b in by_level), default=1)))
+    cap = min(MW_MAX_WAVES, max(1, comp.opts.mw_max_waves))
+    W = max(1, min(cap, -(-total // crit), max((len(b) for b in by_level), default=1)))
     out = []
     for ks in by_level:
         load = [0] * W

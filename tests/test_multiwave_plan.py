@@ -83,3 +83,17 @@ def test_falling_edge_eval_runs_the_cone_mtasks_only(name, threads):
     assert src.count("if (live) {") == 1 + full + cone_lv
     if name in ("alu", "rv32i"):
         assert n_cone == 0 and cone_lv == 0     # inputs are read by clocked logic only
+
+
+@pytest.mark.parametrize("cap", [2, 5, 8])
+def test_wavefront_cap(cap):
+    """Options.mw_max_waves bounds the plan's wavefronts; the plan still
+    covers every mtask once, after its predecessors."""
+    e = registry.ENTRIES["systolic"]
+    spec = e.make()
+    comp = Compiled(elaborate(spec["verilog"], spec["top"]),
+                    Options(threads=16, bitplane=False, split_reductions=32, mw_max_waves=cap))
+    W, levels = mw_plan(comp)
+    assert W == cap
+    seen = sorted(k for lv in levels for ks in lv for k in ks)
+    assert seen == list(range(len(comp.mtasks)))
