@@ -254,7 +254,10 @@ def split_reductions(design: Design, max_terms: int) -> int:
         w = st.rhs.width
         leaves: List[Expr] = []
         _assoc_leaves(st.rhs, st.rhs.op, w, leaves)
-        if len(leaves) <= max_terms:
+        reads: set = set()
+        expr_reads(st.rhs, reads)
+        # (an assignment that reads its own target stays whole)
+        if len(leaves) <= max_terms or st.lhs.var in reads:
             out.append(p)
             continue
         # partial terms, level by level, until the final assignment has at
